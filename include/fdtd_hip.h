@@ -1,0 +1,167 @@
+/*
+ * fdtd_hip.h -- C ABI of the MI355X (gfx950) Yee FDTD time-step loop.
+ *
+ * This is the boundary a js_fdtd backend binds through N-API (see
+ * special-topics_amd/csrc/napi_addon.c) or any other FFI (the Python harness uses
+ * ctypes).  Plain C types only: pointers, sizes, doubles.  Every entry point returns
+ * FDTD_HIP_OK (0) or FDTD_HIP_ERR (1); on error fdtd_hip_last_error() describes the
+ * failure (thread-local, never NULL), mirroring the reference's error convention
+ * (js_fdtd/src/backends/cuda/native/fdtd_cuda.h:12-13,22).
+ *
+ * Array layouts.  Host arrays passed in or out use the reference layouts: flat,
+ * row-major, k fastest, with the component shapes of
+ * js_fdtd/src/backends/CONTRACT.md:21-41 (Hx (nx+1,ny,nz), Hy (nx,ny+1,nz),
+ * Hz (nx,ny,nz+1), Ex (nx,ny+1,nz+1), Ey (nx+1,ny,nz+1), Ez (nx+1,ny+1,nz)), CPML
+ * Psi/CPsi arrays with the component's shape and the face-axis extent replaced by nc
+ * (js_fdtd/src/cpml.js:153-170,262-275,360-372).  On the device the library keeps its
+ * own layout; callers never see it.
+ */
+#ifndef FDTD_HIP_H
+#define FDTD_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FDTD_HIP_OK 0
+#define FDTD_HIP_ERR 1
+
+/* Arithmetic of the field update.  FP64 reproduces the reference's FP64 WASM path
+ * operation for operation; FP32 keeps fields, coefficients and CPML state in FP32 and
+ * the sampled values, port DFTs and NTFF accumulators in FP64. */
+#define FDTD_HIP_FP64 0
+#define FDTD_HIP_FP32 1
+
+/* What happens to Ey/Ez on the x = max wall (i = nx).  REFERENCE: updated like the
+ * reference's updateE, whose reads of Hz[nx,..] and Hy[nx,..] land in Ex(0,..) (after
+ * this step's bulk update) and in Hz's first plane (js_fdtd/assembly/fdtd_kernels.ts:
+ * 159,185 over the layout of js_fdtd/src/grid.js:136-141).  MATLAB: left to the CPML
+ * terms only, as the original MATLAB solver does. */
+#define FDTD_HIP_BOUNDARY_REFERENCE 0
+#define FDTD_HIP_BOUNDARY_MATLAB 1
+
+/* Problem dimensions.  Replaces FdtdDims (fdtd_cuda.h:15-19), with FP64 spacings and
+ * the precision / boundary selectors. */
+typedef struct {
+    int nx, ny, nz;
+    double dx, dy, dz, dt;
+    int numberOfTimeSteps;
+    int precision;   /* FDTD_HIP_FP64 | FDTD_HIP_FP32 */
+    int boundary;    /* FDTD_HIP_BOUNDARY_REFERENCE | FDTD_HIP_BOUNDARY_MATLAB */
+} FdtdHipDims;
+
+/* Opaque per-problem state (the reference addon keeps one global problem,
+ * fdtd_cuda.cu:118; a handle allows several problems per process). */
+typedef struct FdtdHipContext FdtdHipContext;
+
+/* Accumulated device time per kernel kind while timing is enabled. */
+typedef struct {
+    double ms[4];        /* 0 = H update, 1 = E update, 2 = port epilogue, 3 = NTFF */
+    long long launches[4];
+} FdtdHipKernelTimes;
+
+/* Last error text of the calling thread.  Replaces fdtd_last_error (fdtd_cuda.h:22). */
+const char *fdtd_hip_last_error(void);
+
+/* Number of usable gfx950 devices; 0 (and FDTD_HIP_OK) when none or no driver.
+ * Replaces fdtd_device_count (fdtd_cuda.h:25). */
+int fdtd_hip_device_count(int *count);
+
+/* Allocate device state for one problem; fields and CPML state start at zero.
+ * Replaces fdtd_init (fdtd_cuda.h:28). */
+int fdtd_hip_init(const FdtdHipDims *dims, FdtdHipContext **out);
+
+/* One of the 18 coefficient arrays, numbered in the reference's buffer order
+ * Cexe,Cexhz,Cexhy,Ceye,Ceyhx,Ceyhz,Ceze,Cezhy,Cezhx,Chxh,Chxez,Chxey,Chyh,Chyex,
+ * Chyez,Chzh,Chzey,Chzex (js_fdtd/src/grid.js:144-162); n must equal the owning
+ * component's length.  Replaces fdtd_upload_coeff (fdtd_cuda.h:31), FP64 input.
+ * The _f32 variant takes FP32 input and is valid only for FDTD_HIP_FP32 contexts. */
+int fdtd_hip_upload_coeff(FdtdHipContext *ctx, int which, const double *host, long long n);
+int fdtd_hip_upload_coeff_f32(FdtdHipContext *ctx, int which, const float *host, long long n);
+
+/* One CPML face, faceId 0..5 = xn,xp,yn,yp,zn,zp.  Profiles b_e,a_e,b_m,a_m have nc
+ * entries; CPsi0..3 are the face's two magnetic then two electric CPsi arrays in the
+ * slot order of js_fdtd/src/backends/cuda/cudaBackend.js:86-114 (x: hyx,hzx,eyx,ezx;
+ * y: hxy,hzy,ezy,exy; z: hxz,hyz,exz,eyz) with lengths n0..n3.  m_start / e_start are
+ * the first H / E plane along the face axis (cpml.js:174,249,277-278,374-375).
+ * Replaces fdtd_upload_cpml (fdtd_cuda.h:36-40).  Psi starts at zero. */
+int fdtd_hip_upload_cpml(FdtdHipContext *ctx, int faceId, int nc,
+                         const double *b_e, const double *a_e, const double *b_m, const double *a_m,
+                         const double *CPsi0, long long n0, const double *CPsi1, long long n1,
+                         const double *CPsi2, long long n2, const double *CPsi3, long long n3,
+                         int m_start, int e_start);
+
+/* A lumped voltage source on `count` E edges of component dir (0=x,1=y,2=z): after
+ * the E update of step ts, E[idx[n]] += coef[n] * waveform[ts] (sources.js:271-288).
+ * May be called once per source.  Replaces fdtd_upload_vsource (fdtd_cuda.h:43-44). */
+int fdtd_hip_upload_vsource(FdtdHipContext *ctx, int dir, const int *idx, const double *coef,
+                            int count, const double *waveform, int nSteps);
+
+/* A sampled voltage: V = Csvf * (sum of E[idx[n]] in list order)
+ * (sampling.js:295-307).  Replaces fdtd_upload_svoltage (fdtd_cuda.h:47). */
+int fdtd_hip_upload_svoltage(FdtdHipContext *ctx, int dir, const int *idx, int count, double Csvf);
+
+/* A sampled current as an ordered list of H terms: I = sum(weight[n] * H_comp[n][index[n]])
+ * left to right, negated when negate != 0 (the Ampere loop of sampling.js:204-245).
+ * Replaces fdtd_upload_scurrent (fdtd_cuda.h:50-53), which took three unordered
+ * gather lists; the ordered list keeps the FP64 sum identical to the reference's. */
+int fdtd_hip_upload_scurrent(FdtdHipContext *ctx, const int *comp, const int *index,
+                             const double *weight, int count, int negate);
+
+/* Huygens box and DFT kernels of the NTFF transform (farfield.js:28-78,90-248).
+ * box = {li,lj,lk,ui,uj,uk} (1-based, as in initFarfield); phase = [nSteps][nFreq][4]
+ * of cos(w dt (ts+1/2)), -sin(..), cos(w dt (ts+1)), -sin(..) as the host evaluates
+ * them.  Accumulators start at zero.  No counterpart in the reference ABI (its CUDA
+ * backend reads all six fields back every step, cudaBackend.js:254-270). */
+int fdtd_hip_upload_farfield(FdtdHipContext *ctx, int nFreq, const int *box,
+                             const double *phase, int nSteps);
+
+/* Running FP64 DFTs of every sampled voltage and current at nFreq frequencies:
+ * X(f) = dt * sum_n x_n exp(-j 2 pi f ((n+1) dt + shift)), shift 0 for voltages and
+ * -dt/2 for currents (dft.js:17-38 as called at index.js:69-70).  No counterpart in
+ * the reference ABI (the reference transforms the traces after the loop). */
+int fdtd_hip_upload_port_dft(FdtdHipContext *ctx, int nFreq, const double *freqHz);
+
+/* Run steps [tsStart, tsStart+count).  svOut: [nSampledVoltages][count],
+ * siOut: [nSampledCurrents][count], FP64, either may be NULL.
+ * Replaces fdtd_run_batch (fdtd_cuda.h:57). */
+int fdtd_hip_run_batch(FdtdHipContext *ctx, int tsStart, int count, double *svOut, double *siOut);
+
+/* Copy a field (which: 0=Hx,1=Hy,2=Hz,3=Ex,4=Ey,5=Ez) to host in the reference
+ * layout; n must equal the component's length.  Replaces fdtd_read_field
+ * (fdtd_cuda.h:60), FP64 output. */
+int fdtd_hip_read_field(FdtdHipContext *ctx, int which, double *host, long long n);
+
+/* Overwrite a field from a host array in the reference layout (checkpoint restore
+ * and test set-up). */
+int fdtd_hip_write_field(FdtdHipContext *ctx, int which, const double *host, long long n);
+
+/* Copy a CPML Psi array (faceId, slot 0..3 as in fdtd_hip_upload_cpml) to host, or
+ * overwrite it from host (checkpoint restore and test set-up). */
+int fdtd_hip_read_psi(FdtdHipContext *ctx, int faceId, int slot, double *host, long long n);
+int fdtd_hip_write_psi(FdtdHipContext *ctx, int faceId, int slot, const double *host, long long n);
+
+/* Copy NTFF accumulator `which` (0..23 in the order cjyxp,cjzxp,cmyxp,cmzxp, cjyxn,...,
+ * cmyzn: faces xp,xn,yp,yn,zp,zn, per face the J then M tangential components as
+ * declared at farfield.js:63-76) as re[] and im[] of nFreq*faceSize entries. */
+int fdtd_hip_read_farfield(FdtdHipContext *ctx, int which, double *re, double *im, long long n);
+
+/* Copy the running port DFTs: [nSampledVoltages + nSampledCurrents][nFreq] complex,
+ * voltages first, already multiplied by dt. */
+int fdtd_hip_read_port_dft(FdtdHipContext *ctx, double *re, double *im, long long n);
+
+/* Per-kernel device timing with HIP events on the library's stream. */
+int fdtd_hip_set_timing(FdtdHipContext *ctx, int enable);
+int fdtd_hip_get_kernel_times(FdtdHipContext *ctx, FdtdHipKernelTimes *out);
+
+/* Block until all queued work of the context has finished. */
+int fdtd_hip_synchronize(FdtdHipContext *ctx);
+
+/* Free all device state.  Replaces fdtd_destroy (fdtd_cuda.h:63). */
+void fdtd_hip_destroy(FdtdHipContext *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FDTD_HIP_H */

@@ -1,0 +1,636 @@
+// fdtd_hip.hip -- host side of the gfx950 Yee time step behind the C ABI of
+// include/fdtd_hip.h.  Owns device memory, converts between the reference array
+// layouts and the device layout (see fdtd_kernels.hpp), and sequences the per-step
+// kernels on one HIP stream.
+
+#include "../../include/fdtd_hip.h"
+#include "fdtd_kernels.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+using namespace fdtd;
+
+namespace {
+
+thread_local std::string g_last_error = "no error";
+
+int fail(const char *fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_last_error = buf;
+  return FDTD_HIP_ERR;
+}
+
+#define HIP_TRY(expr)                                                                  \
+  do {                                                                                 \
+    hipError_t _e = (expr);                                                            \
+    if (_e != hipSuccess) return fail("%s: %s (%s:%d)", #expr, hipGetErrorString(_e),  \
+                                      __FILE__, __LINE__);                             \
+  } while (0)
+
+// Component shapes as offsets from (nx, ny, nz), indexed Hx,Hy,Hz,Ex,Ey,Ez.
+const int kExt[6][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {0, 1, 1}, {1, 0, 1}, {1, 1, 0}};
+// Owning component of each coefficient id (reference buffer order).
+const int kCoeffOwner[NCOEFF] = {3, 3, 3, 4, 4, 4, 5, 5, 5, 0, 0, 0, 1, 1, 1, 2, 2, 2};
+// CPML slot -> component, per face axis (x: Hy,Hz,Ey,Ez; y: Hx,Hz,Ez,Ex; z: Hx,Hy,Ex,Ey).
+const int kSlotComp[3][4] = {{1, 2, 4, 5}, {0, 2, 5, 3}, {0, 1, 3, 4}};
+
+struct DevBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+};
+
+}  // namespace
+
+struct FdtdHipContext {
+  FdtdHipDims d{};
+  int SJ = 0, SI = 0;
+  long long ntot = 0;           // elements per uniform array
+  size_t esz = 8;               // bytes per field element
+  int device = 0;
+  hipStream_t stream = nullptr;
+  void *fields = nullptr;       // 6 uniform arrays
+  void *coeffs = nullptr;       // 18 uniform arrays
+  size_t stride_bytes = 0;      // bytes between consecutive uniform arrays
+
+  struct HostFace {
+    int nc = 0, m_start = 0, e_start = 0;
+    void *prof = nullptr;       // b_e, a_e, b_m, a_m (T, nc each)
+    void *psi[4] = {};
+    void *cpsi[4] = {};
+    long long n[4] = {};
+  } face[6];
+
+  std::vector<void *> small_allocs;   // port tables, traces
+  PortState ports{};
+  int traces_ready = 0;
+
+  int nff = 0;
+  FarfieldState ff{};
+  double *ff_buf = nullptr;
+  long long ff_sizes[24] = {};
+  double *phase = nullptr;
+  int phase_steps = 0;
+
+  int ndft = 0;
+  double *dft_w = nullptr, *dft_acc = nullptr;
+
+  int timing = 0;
+  hipEvent_t ev[8] = {};
+  FdtdHipKernelTimes times{};
+
+  void *field_ptr(int c) const { return (char *)fields + c * stride_bytes; }
+  void *coeff_ptr(int w) const { return (char *)coeffs + w * stride_bytes; }
+  void comp_dims(int c, int out[3]) const {
+    out[0] = d.nx + kExt[c][0];
+    out[1] = d.ny + kExt[c][1];
+    out[2] = d.nz + kExt[c][2];
+  }
+  long long comp_len(int c) const {
+    int t[3];
+    comp_dims(c, t);
+    return (long long)t[0] * t[1] * t[2];
+  }
+  // Device offset of a reference-layout flat index of component c.
+  long long to_dev(int c, long long f) const {
+    int t[3];
+    comp_dims(c, t);
+    long long plane = (long long)t[1] * t[2];
+    long long i = f / plane, r = f - i * plane, j = r / t[2], k = r - j * t[2];
+    return i * SI + j * SJ + k;
+  }
+  bool fp32() const { return d.precision == FDTD_HIP_FP32; }
+};
+
+namespace {
+
+template <typename T>
+Grid<T> make_grid(const FdtdHipContext *c) {
+  Grid<T> g;
+  g.nx = c->d.nx; g.ny = c->d.ny; g.nz = c->d.nz;
+  g.SJ = c->SJ; g.SI = c->SI;
+  g.Hx = (T *)c->field_ptr(0); g.Hy = (T *)c->field_ptr(1); g.Hz = (T *)c->field_ptr(2);
+  g.Ex = (T *)c->field_ptr(3); g.Ey = (T *)c->field_ptr(4); g.Ez = (T *)c->field_ptr(5);
+  for (int w = 0; w < NCOEFF; ++w) g.C[w] = (const T *)c->coeff_ptr(w);
+  return g;
+}
+
+template <typename T>
+FaceSet<T> make_faces(const FdtdHipContext *c) {
+  FaceSet<T> fs;
+  for (int f = 0; f < 6; ++f) {
+    const auto &h = c->face[f];
+    Face<T> &F = fs.f[f];
+    F.nc = h.nc; F.m_start = h.m_start; F.e_start = h.e_start;
+    const T *prof = (const T *)h.prof;
+    F.b_e = prof; F.a_e = prof ? prof + h.nc : nullptr;
+    F.b_m = prof ? prof + 2 * h.nc : nullptr; F.a_m = prof ? prof + 3 * h.nc : nullptr;
+    for (int s = 0; s < 4; ++s) { F.psi[s] = (T *)h.psi[s]; F.cpsi[s] = (const T *)h.cpsi[s]; }
+  }
+  return fs;
+}
+
+// Reference layout (component c, FP64 or FP32 source) -> uniform device array of T.
+template <typename T, typename S>
+void pack_uniform(const FdtdHipContext *c, int comp, const S *src, std::vector<T> &dst) {
+  dst.assign(c->ntot, T(0));
+  int t[3];
+  c->comp_dims(comp, t);
+  for (long long i = 0; i < t[0]; ++i)
+    for (long long j = 0; j < t[1]; ++j) {
+      const S *row = src + (i * t[1] + j) * t[2];
+      T *out = dst.data() + i * c->SI + j * c->SJ;
+      for (long long k = 0; k < t[2]; ++k) out[k] = (T)row[k];
+    }
+}
+
+template <typename T>
+void unpack_uniform(const FdtdHipContext *c, int comp, const std::vector<T> &src, double *dst) {
+  int t[3];
+  c->comp_dims(comp, t);
+  for (long long i = 0; i < t[0]; ++i)
+    for (long long j = 0; j < t[1]; ++j) {
+      const T *row = src.data() + i * c->SI + j * c->SJ;
+      double *out = dst + (i * t[1] + j) * t[2];
+      for (long long k = 0; k < t[2]; ++k) out[k] = (double)row[k];
+    }
+}
+
+int dev_alloc(FdtdHipContext *c, void **p, size_t bytes) {
+  HIP_TRY(hipMalloc(p, bytes ? bytes : 16));
+  HIP_TRY(hipMemsetAsync(*p, 0, bytes ? bytes : 16, c->stream));
+  return FDTD_HIP_OK;
+}
+
+template <typename T>
+int upload_vec(FdtdHipContext *c, void *dst, const std::vector<T> &v) {
+  HIP_TRY(hipMemcpyAsync(dst, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return FDTD_HIP_OK;
+}
+
+template <typename S>
+int upload_coeff_impl(FdtdHipContext *c, int which, const S *host, long long n) {
+  if (!c) return fail("upload_coeff: null context");
+  if (which < 0 || which >= NCOEFF) return fail("upload_coeff: coefficient id %d out of range", which);
+  const int comp = kCoeffOwner[which];
+  if (n != c->comp_len(comp))
+    return fail("upload_coeff: array %d has %lld elements, expected %lld", which, n, c->comp_len(comp));
+  if (c->fp32()) {
+    std::vector<float> v;
+    pack_uniform(c, comp, host, v);
+    return upload_vec(c, c->coeff_ptr(which), v);
+  }
+  std::vector<double> v;
+  pack_uniform(c, comp, host, v);
+  return upload_vec(c, c->coeff_ptr(which), v);
+}
+
+template <typename T>
+int upload_small(FdtdHipContext *c, const std::vector<T> &v, T **out) {
+  void *p = nullptr;
+  if (dev_alloc(c, &p, v.size() * sizeof(T))) return FDTD_HIP_ERR;
+  c->small_allocs.push_back(p);
+  if (!v.empty() && upload_vec(c, p, v)) return FDTD_HIP_ERR;
+  *out = (T *)p;
+  return FDTD_HIP_OK;
+}
+
+int ensure_traces(FdtdHipContext *c) {
+  if (c->traces_ready) return FDTD_HIP_OK;
+  const int steps = c->d.numberOfTimeSteps;
+  std::vector<double> zv((size_t)std::max(1, c->ports.nsv) * steps, 0.0);
+  std::vector<double> zi((size_t)std::max(1, c->ports.nsi) * steps, 0.0);
+  if (upload_small(c, zv, &c->ports.vtr) || upload_small(c, zi, &c->ports.itr)) return FDTD_HIP_ERR;
+  c->ports.steps = steps;
+  c->ports.dt = c->d.dt;
+  c->traces_ready = 1;
+  return FDTD_HIP_OK;
+}
+
+template <typename T>
+int launch_step(FdtdHipContext *c, int ts) {
+  const Grid<T> g = make_grid<T>(c);
+  const FaceSet<T> fs = make_faces<T>(c);
+  const dim3 block(256);
+  const int ghost = c->d.boundary == FDTD_HIP_BOUNDARY_REFERENCE;
+  const dim3 gh((c->SI + 255) / 256, c->d.nx + 1);
+  const dim3 ge((c->SI + 255) / 256, ghost ? c->d.nx : c->d.nx + 1);
+  const bool t = c->timing;
+
+  if (t) HIP_TRY(hipEventRecord(c->ev[0], c->stream));
+  hipLaunchKernelGGL(k_update_h<T>, gh, block, 0, c->stream, g, fs);
+  if (t) HIP_TRY(hipEventRecord(c->ev[1], c->stream));
+  hipLaunchKernelGGL(k_update_e<T>, ge, block, 0, c->stream, g, fs, ghost);
+  if (t) HIP_TRY(hipEventRecord(c->ev[2], c->stream));
+  if (c->ports.nvs + c->ports.nsv + c->ports.nsi > 0) {
+    PortState ps = c->ports;
+    ps.ndft = c->ndft; ps.w = c->dft_w; ps.dft = c->dft_acc;
+    hipLaunchKernelGGL(k_port_epilogue<T>, dim3(1), block, 0, c->stream, g, ps, ts);
+  }
+  if (t) HIP_TRY(hipEventRecord(c->ev[3], c->stream));
+  if (c->nff > 0) {
+    const long total = 2 * ((long)c->ff.nj * c->ff.nk + (long)c->ff.ni * c->ff.nk + (long)c->ff.ni * c->ff.nj);
+    hipLaunchKernelGGL(k_ntff_direct<T>, dim3((unsigned)((total + 255) / 256)), block, 0, c->stream, g, c->ff, ts);
+  }
+  if (t) HIP_TRY(hipEventRecord(c->ev[4], c->stream));
+  HIP_TRY(hipGetLastError());
+  if (t) {
+    HIP_TRY(hipEventSynchronize(c->ev[4]));
+    for (int q = 0; q < 4; ++q) {
+      float ms = 0.f;
+      HIP_TRY(hipEventElapsedTime(&ms, c->ev[q], c->ev[q + 1]));
+      c->times.ms[q] += ms;
+      c->times.launches[q] += 1;
+    }
+  }
+  return FDTD_HIP_OK;
+}
+
+}  // namespace
+
+// =================================================================== C ABI ====
+
+extern "C" {
+
+const char *fdtd_hip_last_error(void) { return g_last_error.c_str(); }
+
+int fdtd_hip_device_count(int *count) {
+  if (!count) return fail("device_count: null pointer");
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
+  int usable = 0;
+  for (int i = 0; i < n; ++i) {
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, i) == hipSuccess && std::strncmp(prop.gcnArchName, "gfx950", 6) == 0)
+      ++usable;
+  }
+  *count = usable;
+  return FDTD_HIP_OK;
+}
+
+int fdtd_hip_init(const FdtdHipDims *dims, FdtdHipContext **out) {
+  if (!dims || !out) return fail("init: null argument");
+  *out = nullptr;
+  const FdtdHipDims &d = *dims;
+  if (d.nx < 2 || d.ny < 2 || d.nz < 2) return fail("init: grid %dx%dx%d too small", d.nx, d.ny, d.nz);
+  if (d.precision != FDTD_HIP_FP64 && d.precision != FDTD_HIP_FP32) return fail("init: unknown precision %d", d.precision);
+  if (d.boundary != FDTD_HIP_BOUNDARY_REFERENCE && d.boundary != FDTD_HIP_BOUNDARY_MATLAB)
+    return fail("init: unknown boundary mode %d", d.boundary);
+  if (d.numberOfTimeSteps < 1) return fail("init: numberOfTimeSteps must be positive");
+  const long long ntot = (long long)(d.nx + 1) * (d.ny + 1) * (d.nz + 1);
+  if (ntot >= (1LL << 31) - 1) return fail("init: %lld cells exceed the 32-bit device indexing", ntot);
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail("init: no HIP device");
+
+  auto *c = new FdtdHipContext();
+  c->d = d;
+  c->SJ = d.nz + 1;
+  c->SI = (d.ny + 1) * (d.nz + 1);
+  c->ntot = ntot;
+  c->esz = c->fp32() ? 4 : 8;
+  c->stride_bytes = ((size_t)ntot * c->esz + 255) / 256 * 256;
+  if (hipGetDevice(&c->device) != hipSuccess) c->device = 0;
+  int rc = FDTD_HIP_OK;
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) rc = fail("init: stream creation failed");
+  if (!rc) rc = dev_alloc(c, &c->fields, 6 * c->stride_bytes);
+  if (!rc) rc = dev_alloc(c, &c->coeffs, NCOEFF * c->stride_bytes);
+  for (int q = 0; !rc && q < 8; ++q)
+    if (hipEventCreate(&c->ev[q]) != hipSuccess) rc = fail("init: event creation failed");
+  if (!rc && hipStreamSynchronize(c->stream) != hipSuccess) rc = fail("init: synchronize failed");
+  if (rc) {
+    fdtd_hip_destroy(c);
+    return rc;
+  }
+  *out = c;
+  return FDTD_HIP_OK;
+}
+
+int fdtd_hip_upload_coeff(FdtdHipContext *c, int which, const double *host, long long n) {
+  if (!host) return fail("upload_coeff: null array");
+  return upload_coeff_impl(c, which, host, n);
+}
+
+int fdtd_hip_upload_coeff_f32(FdtdHipContext *c, int which, const float *host, long long n) {
+  if (!host) return fail("upload_coeff_f32: null array");
+  if (c && !c->fp32()) return fail("upload_coeff_f32: context is FP64; FP32 input would lose the reference bits");
+  return upload_coeff_impl(c, which, host, n);
+}
+
+int fdtd_hip_upload_cpml(FdtdHipContext *c, int faceId, int nc, const double *b_e, const double *a_e,
+                         const double *b_m, const double *a_m, const double *CPsi0, long long n0,
+                         const double *CPsi1, long long n1, const double *CPsi2, long long n2,
+                         const double *CPsi3, long long n3, int m_start, int e_start) {
+  if (!c) return fail("upload_cpml: null context");
+  if (faceId < 0 || faceId > 5) return fail("upload_cpml: face id %d out of range", faceId);
+  const int axis = faceId / 2;
+  const int n = axis == 0 ? c->d.nx : (axis == 1 ? c->d.ny : c->d.nz);
+  if (nc <= 0 || 2 * nc >= n) return fail("upload_cpml: nc=%d invalid for an axis of %d cells", nc, n);
+  if (m_start < 0 || m_start + nc > n) return fail("upload_cpml: H planes [%d,%d) outside [0,%d)", m_start, m_start + nc, n);
+  if (e_start < 1 || e_start + nc > n) return fail("upload_cpml: E planes [%d,%d) outside [1,%d)", e_start, e_start + nc, n);
+  if (!b_e || !a_e || !b_m || !a_m) return fail("upload_cpml: null profile");
+  auto &F = c->face[faceId];
+  if (F.nc) return fail("upload_cpml: face %d already uploaded", faceId);
+  const double *cp[4] = {CPsi0, CPsi1, CPsi2, CPsi3};
+  const long long nn[4] = {n0, n1, n2, n3};
+  for (int s = 0; s < 4; ++s) {
+    int t[3];
+    c->comp_dims(kSlotComp[axis][s], t);
+    t[axis] = nc;
+    const long long want = (long long)t[0] * t[1] * t[2];
+    if (!cp[s] || nn[s] != want) return fail("upload_cpml: face %d slot %d has %lld elements, expected %lld", faceId, s, nn[s], want);
+  }
+  F.m_start = m_start;
+  F.e_start = e_start;
+  const double *pr[4] = {b_e, a_e, b_m, a_m};
+  if (c->fp32()) {
+    std::vector<float> prof(4 * nc);
+    for (int q = 0; q < 4; ++q) for (int ci = 0; ci < nc; ++ci) prof[q * nc + ci] = (float)pr[q][ci];
+    float *p = nullptr;
+    if (upload_small(c, prof, &p)) return FDTD_HIP_ERR;
+    F.prof = p;
+  } else {
+    std::vector<double> prof(4 * nc);
+    for (int q = 0; q < 4; ++q) for (int ci = 0; ci < nc; ++ci) prof[q * nc + ci] = pr[q][ci];
+    double *p = nullptr;
+    if (upload_small(c, prof, &p)) return FDTD_HIP_ERR;
+    F.prof = p;
+  }
+  for (int s = 0; s < 4; ++s) {
+    F.n[s] = nn[s];
+    if (dev_alloc(c, &F.psi[s], nn[s] * c->esz) || dev_alloc(c, &F.cpsi[s], nn[s] * c->esz)) return FDTD_HIP_ERR;
+    if (c->fp32()) {
+      std::vector<float> v(cp[s], cp[s] + nn[s]);
+      if (upload_vec(c, F.cpsi[s], v)) return FDTD_HIP_ERR;
+    } else {
+      std::vector<double> v(cp[s], cp[s] + nn[s]);
+      if (upload_vec(c, F.cpsi[s], v)) return FDTD_HIP_ERR;
+    }
+  }
+  F.nc = nc;
+  return FDTD_HIP_OK;
+}
+
+int fdtd_hip_upload_vsource(FdtdHipContext *c, int dir, const int *idx, const double *coef, int count,
+                            const double *waveform, int nSteps) {
+  if (!c) return fail("upload_vsource: null context");
+  if (dir < 0 || dir > 2) return fail("upload_vsource: direction %d out of range", dir);
+  if (c->ports.nvs >= MAX_PORT_OBJS) return fail("upload_vsource: at most %d sources", MAX_PORT_OBJS);
+  if (count < 0 || (count > 0 && (!idx || !coef))) return fail("upload_vsource: bad edge list");
+  if (!waveform || nSteps < c->d.numberOfTimeSteps)
+    return fail("upload_vsource: waveform has %d samples, need %d", nSteps, c->d.numberOfTimeSteps);
+  const int comp = 3 + dir;
+  std::vector<int> di(count);
+  for (int n = 0; n < count; ++n) {
+    if (idx[n] < 0 || idx[n] >= c->comp_len(comp)) return fail("upload_vsource: edge index %d outside E%c", idx[n], "xyz"[dir]);
+    for (int m = 0; m < n; ++m)
+      if (idx[m] == idx[n]) return fail("upload_vsource: duplicate edge index %d", idx[n]);
+    di[n] = (int)c->to_dev(comp, idx[n]);
+  }
+  VSourceDev &v = c->ports.vs[c->ports.nvs];
+  v.dir = dir; v.count = count;
+  int *pi = nullptr; double *pc = nullptr, *pw = nullptr;
+  if (upload_small(c, di, &pi)) return FDTD_HIP_ERR;
+  if (upload_small(c, std::vector<double>(coef, coef + count), &pc)) return FDTD_HIP_ERR;
+  if (upload_small(c, std::vector<double>(waveform, waveform + c->d.numberOfTimeSteps), &pw)) return FDTD_HIP_ERR;
+  v.idx = pi; v.coef = pc; v.wave = pw;
+  c->ports.nvs++;
+  return FDTD_HIP_OK;
+}
+
+int fdtd_hip_upload_svoltage(FdtdHipContext *c, int dir, const int *idx, int count, double Csvf) {
+  if (!c) return fail("upload_svoltage: null context");
+  if (dir < 0 || dir > 2) return fail("upload_svoltage: direction %d out of range", dir);
+  if (c->ports.nsv >= MAX_PORT_OBJS) return fail("upload_svoltage: at most %d observers", MAX_PORT_OBJS);
+  if (c->traces_ready) return fail("upload_svoltage: observers must be uploaded before the first step");
+  if (count < 0 || (count > 0 && !idx)) return fail("upload_svoltage: bad edge list");
+  const int comp = 3 + dir;
+  std::vector<int> di(count);
+  for (int n = 0; n < count; ++n) {
+    if (idx[n] < 0 || idx[n] >= c->comp_len(comp)) return fail("upload_svoltage: edge index %d outside E%c", idx[n], "xyz"[dir]);
+    di[n] = (int)c->to_dev(comp, idx[n]);
+  }
+  SVoltageDev &s = c->ports.sv[c->ports.nsv];
+  int *pi = nullptr;
+  if (upload_small(c, di, &pi)) return FDTD_HIP_ERR;
+  s.dir = dir; s.count = count; s.idx = pi; s.csvf = Csvf;
+  c->ports.nsv++;
+  return FDTD_HIP_OK;
+}
+
+int fdtd_hip_upload_scurrent(FdtdHipContext *c, const int *comp, const int *index, const double *weight,
+                             int count, int negate) {
+  if (!c) return fail("upload_scurrent: null context");
+  if (c->ports.nsi >= MAX_PORT_OBJS) return fail("upload_scurrent: at most %d observers", MAX_PORT_OBJS);
+  if (c->traces_ready) return fail("upload_scurrent: observers must be uploaded before the first step");
+  if (count < 0 || (count > 0 && (!comp || !index || !weight))) return fail("upload_scurrent: bad term list");
+  std::vector<int> dc(count), di(count);
+  for (int n = 0; n < count; ++n) {
+    if (comp[n] < 0 || comp[n] > 2) return fail("upload_scurrent: term %d names H component %d", n, comp[n]);
+    if (index[n] < 0 || index[n] >= c->comp_len(comp[n])) return fail("upload_scurrent: term %d index %d out of range", n, index[n]);
+    dc[n] = comp[n];
+    di[n] = (int)c->to_dev(comp[n], index[n]);
+  }
+  SCurrentDev &s = c->ports.si[c->ports.nsi];
+  int *pc = nullptr, *pi = nullptr; double *pw = nullptr;
+  if (upload_small(c, dc, &pc) || upload_small(c, di, &pi) ||
+      upload_small(c, std::vector<double>(weight, weight + count), &pw)) return FDTD_HIP_ERR;
+  s.count = count; s.negate = negate ? 1 : 0; s.comp = pc; s.idx = pi; s.w = pw;
+  c->ports.nsi++;
+  return FDTD_HIP_OK;
+}
+
+int fdtd_hip_upload_farfield(FdtdHipContext *c, int nFreq, const int *box, const double *phase, int nSteps) {
+  if (!c) return fail("upload_farfield: null context");
+  if (c->nff) return fail("upload_farfield: already uploaded");
+  if (nFreq <= 0 || !box || !phase) return fail("upload_farfield: bad arguments");
+  if (nSteps < c->d.numberOfTimeSteps) return fail("upload_farfield: phase table has %d steps, need %d", nSteps, c->d.numberOfTimeSteps);
+  const int li = box[0], lj = box[1], lk = box[2], ui = box[3], uj = box[4], uk = box[5];
+  // Reads reach planes LI-1..UI (+1 along the face) on every axis.
+  if (li < 2 || lj < 2 || lk < 2 || ui > c->d.nx || uj > c->d.ny || uk > c->d.nz || ui <= li || uj <= lj || uk <= lk)
+    return fail("upload_farfield: box [%d,%d]x[%d,%d]x[%d,%d] does not fit the %dx%dx%d grid",
+                li, ui, lj, uj, lk, uk, c->d.nx, c->d.ny, c->d.nz);
+  FarfieldState &f = c->ff;
+  f.nfreq = nFreq;
+  f.LI = li - 1; f.LJ = lj - 1; f.LK = lk - 1; f.UI = ui - 1; f.UJ = uj - 1; f.UK = uk - 1;
+  f.ni = ui - li; f.nj = uj - lj; f.nk = uk - lk;
+  long long sz[3] = {(long long)f.nj * f.nk, (long long)f.ni * f.nk, (long long)f.ni * f.nj};
+  long long total = 0;
+  for (int a = 0; a < 24; ++a) { c->ff_sizes[a] = sz[a / 8] * nFreq; total += 2 * c->ff_sizes[a]; }
+  void *p = nullptr;
+  if (dev_alloc(c, &p, total * sizeof(double))) return FDTD_HIP_ERR;
+  c->ff_buf = (double *)p;
+  f.acc = c->ff_buf;
+  double *ph = nullptr;
+  if (upload_small(c, std::vector<double>(phase, phase + (size_t)c->d.numberOfTimeSteps * nFreq * 4), &ph))
+    return FDTD_HIP_ERR;
+  f.phase = ph;
+  c->nff = nFreq;
+  return FDTD_HIP_OK;
+}
+
+int fdtd_hip_upload_port_dft(FdtdHipContext *c, int nFreq, const double *freqHz) {
+  if (!c) return fail("upload_port_dft: null context");
+  if (c->ndft) return fail("upload_port_dft: already uploaded");
+  if (nFreq <= 0 || !freqHz) return fail("upload_port_dft: bad arguments");
+  std::vector<double> w(nFreq);
+  for (int f = 0; f < nFreq; ++f) w[f] = 2 * M_PI * freqHz[f];
+  if (upload_small(c, w, &c->dft_w)) return FDTD_HIP_ERR;
+  const int ntr = std::max(1, c->ports.nsv + c->ports.nsi);
+  if (upload_small(c, std::vector<double>((size_t)2 * ntr * nFreq, 0.0), &c->dft_acc)) return FDTD_HIP_ERR;
+  c->ndft = nFreq;
+  return FDTD_HIP_OK;
+}
+
+int fdtd_hip_run_batch(FdtdHipContext *c, int tsStart, int count, double *svOut, double *siOut) {
+  if (!c) return fail("run_batch: null context");
+  if (tsStart < 0 || count < 0 || tsStart + count > c->d.numberOfTimeSteps)
+    return fail("run_batch: steps [%d,%d) outside [0,%d)", tsStart, tsStart + count, c->d.numberOfTimeSteps);
+  if (c->ndft && c->ports.nsv + c->ports.nsi == 0) return fail("run_batch: port DFT without observers");
+  if (ensure_traces(c)) return FDTD_HIP_ERR;
+  for (int s = 0; s < count; ++s) {
+    const int rc = c->fp32() ? launch_step<float>(c, tsStart + s) : launch_step<double>(c, tsStart + s);
+    if (rc) return rc;
+  }
+  for (int o = 0; svOut && o < c->ports.nsv; ++o)
+    HIP_TRY(hipMemcpyAsync(svOut + (size_t)o * count, c->ports.vtr + (size_t)o * c->ports.steps + tsStart,
+                           count * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  for (int o = 0; siOut && o < c->ports.nsi; ++o)
+    HIP_TRY(hipMemcpyAsync(siOut + (size_t)o * count, c->ports.itr + (size_t)o * c->ports.steps + tsStart,
+                           count * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return FDTD_HIP_OK;
+}
+
+int fdtd_hip_read_field(FdtdHipContext *c, int which, double *host, long long n) {
+  if (!c || !host) return fail("read_field: null argument");
+  if (which < 0 || which > 5) return fail("read_field: field id %d out of range", which);
+  if (n != c->comp_len(which)) return fail("read_field: %lld elements, expected %lld", n, c->comp_len(which));
+  if (c->fp32()) {
+    std::vector<float> v(c->ntot);
+    HIP_TRY(hipMemcpyAsync(v.data(), c->field_ptr(which), c->ntot * 4, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    unpack_uniform(c, which, v, host);
+  } else {
+    std::vector<double> v(c->ntot);
+    HIP_TRY(hipMemcpyAsync(v.data(), c->field_ptr(which), c->ntot * 8, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    unpack_uniform(c, which, v, host);
+  }
+  return FDTD_HIP_OK;
+}
+
+int fdtd_hip_write_field(FdtdHipContext *c, int which, const double *host, long long n) {
+  if (!c || !host) return fail("write_field: null argument");
+  if (which < 0 || which > 5) return fail("write_field: field id %d out of range", which);
+  if (n != c->comp_len(which)) return fail("write_field: %lld elements, expected %lld", n, c->comp_len(which));
+  if (c->fp32()) {
+    std::vector<float> v;
+    pack_uniform(c, which, host, v);
+    return upload_vec(c, c->field_ptr(which), v);
+  }
+  std::vector<double> v;
+  pack_uniform(c, which, host, v);
+  return upload_vec(c, c->field_ptr(which), v);
+}
+
+static int psi_check(FdtdHipContext *c, int faceId, int slot, long long n, const char *who) {
+  if (!c) return fail("%s: null context", who);
+  if (faceId < 0 || faceId > 5 || slot < 0 || slot > 3) return fail("%s: face %d slot %d out of range", who, faceId, slot);
+  if (!c->face[faceId].nc) return fail("%s: face %d not uploaded", who, faceId);
+  if (n != c->face[faceId].n[slot]) return fail("%s: %lld elements, expected %lld", who, n, c->face[faceId].n[slot]);
+  return FDTD_HIP_OK;
+}
+
+int fdtd_hip_read_psi(FdtdHipContext *c, int faceId, int slot, double *host, long long n) {
+  if (!host) return fail("read_psi: null array");
+  if (psi_check(c, faceId, slot, n, "read_psi")) return FDTD_HIP_ERR;
+  if (c->fp32()) {
+    std::vector<float> v(n);
+    HIP_TRY(hipMemcpyAsync(v.data(), c->face[faceId].psi[slot], n * 4, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    for (long long q = 0; q < n; ++q) host[q] = v[q];
+  } else {
+    HIP_TRY(hipMemcpyAsync(host, c->face[faceId].psi[slot], n * 8, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+  }
+  return FDTD_HIP_OK;
+}
+
+int fdtd_hip_write_psi(FdtdHipContext *c, int faceId, int slot, const double *host, long long n) {
+  if (!host) return fail("write_psi: null array");
+  if (psi_check(c, faceId, slot, n, "write_psi")) return FDTD_HIP_ERR;
+  if (c->fp32()) return upload_vec(c, c->face[faceId].psi[slot], std::vector<float>(host, host + n));
+  return upload_vec(c, c->face[faceId].psi[slot], std::vector<double>(host, host + n));
+}
+
+int fdtd_hip_read_farfield(FdtdHipContext *c, int which, double *re, double *im, long long n) {
+  if (!c || !re || !im) return fail("read_farfield: null argument");
+  if (!c->nff) return fail("read_farfield: no far-field box uploaded");
+  if (which < 0 || which >= 24) return fail("read_farfield: accumulator %d out of range", which);
+  if (n != c->ff_sizes[which]) return fail("read_farfield: %lld elements, expected %lld", n, c->ff_sizes[which]);
+  const FarfieldState &f = c->ff;
+  const double *re_d = f.acc + ff_offset(which, f.nfreq, (long)f.nj * f.nk, (long)f.ni * f.nk, (long)f.ni * f.nj);
+  HIP_TRY(hipMemcpyAsync(re, re_d, n * 8, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipMemcpyAsync(im, re_d + n, n * 8, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return FDTD_HIP_OK;
+}
+
+int fdtd_hip_read_port_dft(FdtdHipContext *c, double *re, double *im, long long n) {
+  if (!c || !re || !im) return fail("read_port_dft: null argument");
+  if (!c->ndft) return fail("read_port_dft: no port DFT uploaded");
+  const long long want = (long long)(c->ports.nsv + c->ports.nsi) * c->ndft;
+  if (n != want) return fail("read_port_dft: %lld elements, expected %lld", n, want);
+  std::vector<double> v(2 * n);
+  HIP_TRY(hipMemcpyAsync(v.data(), c->dft_acc, 2 * n * 8, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  for (long long q = 0; q < n; ++q) { re[q] = v[2 * q] * c->d.dt; im[q] = v[2 * q + 1] * c->d.dt; }
+  return FDTD_HIP_OK;
+}
+
+int fdtd_hip_set_timing(FdtdHipContext *c, int enable) {
+  if (!c) return fail("set_timing: null context");
+  c->timing = enable ? 1 : 0;
+  c->times = FdtdHipKernelTimes{};
+  return FDTD_HIP_OK;
+}
+
+int fdtd_hip_get_kernel_times(FdtdHipContext *c, FdtdHipKernelTimes *out) {
+  if (!c || !out) return fail("get_kernel_times: null argument");
+  *out = c->times;
+  return FDTD_HIP_OK;
+}
+
+int fdtd_hip_synchronize(FdtdHipContext *c) {
+  if (!c) return fail("synchronize: null context");
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return FDTD_HIP_OK;
+}
+
+void fdtd_hip_destroy(FdtdHipContext *c) {
+  if (!c) return;
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (auto &f : c->face) {
+    for (int s = 0; s < 4; ++s) { (void)hipFree(f.psi[s]); (void)hipFree(f.cpsi[s]); }
+  }
+  for (void *p : c->small_allocs) (void)hipFree(p);
+  (void)hipFree(c->ff_buf);
+  (void)hipFree(c->fields);
+  (void)hipFree(c->coeffs);
+  for (auto &e : c->ev) if (e) (void)hipEventDestroy(e);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+}  // extern "C"

@@ -1,0 +1,403 @@
+/*
+ * napi_addon.c -- thin Node-API (C API, node_api.h) glue between the Node backend
+ * (js/backends/hipBackend.mjs) and the C ABI of include/fdtd_hip.h.
+ *
+ * Plays the role of js_fdtd/src/backends/cuda/native/binding.cpp:1-186: typed arrays
+ * cross zero-copy, every failing library call becomes a thrown JS Error whose message
+ * is "<call>: <fdtd_hip_last_error()>".  A problem is an external handle instead of
+ * the reference addon's process-global state.
+ *
+ *   deviceCount() -> number
+ *   init({nx,ny,nz,dx,dy,dz,dt,numberOfTimeSteps,precision,boundary}) -> handle
+ *   uploadCoeff(h, which, Float64Array)
+ *   uploadCpml(h, faceId, {nc,b_e,a_e,b_m,a_m,CPsi0..CPsi3,m_start,e_start})
+ *   uploadVSource(h, dir, Int32Array idx, Float64Array coef, Float64Array waveform)
+ *   uploadSVoltage(h, dir, Int32Array idx, csvf)
+ *   uploadSCurrent(h, Int32Array comp, Int32Array index, Float64Array weight, negate)
+ *   uploadFarfield(h, nFreq, Int32Array box, Float64Array phase)
+ *   uploadPortDft(h, Float64Array freqHz)
+ *   runBatch(h, tsStart, count, Float64Array|null sv, Float64Array|null si)
+ *   readField(h, which, Float64Array)
+ *   readFarfield(h, which, Float64Array re, Float64Array im)
+ *   readPortDft(h, Float64Array re, Float64Array im)
+ *   setTiming(h, bool); kernelTimes(h) -> {ms:[4], launches:[4]}
+ *   destroy(h)
+ */
+#define NAPI_VERSION 8
+#include <node_api.h>
+#include <stdio.h>
+#include <string.h>
+
+#include "../../include/fdtd_hip.h"
+
+#define CALL(env, expr)                                                    \
+  do {                                                                     \
+    if ((expr) != napi_ok) { throw_napi(env, #expr); return NULL; }        \
+  } while (0)
+
+static void throw_napi(napi_env env, const char *what) {
+  const napi_extended_error_info *info = NULL;
+  napi_get_last_error_info(env, &info);
+  char buf[512];
+  snprintf(buf, sizeof buf, "N-API failure in %s: %s", what,
+           info && info->error_message ? info->error_message : "unknown");
+  napi_throw_error(env, NULL, buf);
+}
+
+static napi_value throw_lib(napi_env env, const char *ctx) {
+  char buf[1200];
+  snprintf(buf, sizeof buf, "%s: %s", ctx, fdtd_hip_last_error());
+  napi_throw_error(env, NULL, buf);
+  return NULL;
+}
+
+static napi_value undefined(napi_env env) {
+  napi_value u;
+  napi_get_undefined(env, &u);
+  return u;
+}
+
+/* Typed array data pointer of the expected element type, or NULL (with a thrown
+ * TypeError) when the value is not one.  allow_null accepts JS null/undefined. */
+static void *typed(napi_env env, napi_value v, napi_typedarray_type want, size_t *len, int allow_null, int *ok) {
+  napi_valuetype vt;
+  *ok = 0;
+  if (len) *len = 0;
+  if (napi_typeof(env, v, &vt) != napi_ok) return NULL;
+  if (allow_null && (vt == napi_null || vt == napi_undefined)) { *ok = 1; return NULL; }
+  bool is_ta = false;
+  napi_is_typedarray(env, v, &is_ta);
+  if (!is_ta) { napi_throw_type_error(env, NULL, "expected a typed array"); return NULL; }
+  napi_typedarray_type t;
+  size_t n;
+  void *data;
+  napi_value ab;
+  size_t off;
+  if (napi_get_typedarray_info(env, v, &t, &n, &data, &ab, &off) != napi_ok) return NULL;
+  if (t != want) {
+    napi_throw_type_error(env, NULL, want == napi_float64_array ? "expected a Float64Array" : "expected an Int32Array");
+    return NULL;
+  }
+  if (len) *len = n;
+  *ok = 1;
+  return data;
+}
+
+static int get_args(napi_env env, napi_callback_info info, size_t want, napi_value *argv) {
+  size_t argc = want;
+  if (napi_get_cb_info(env, info, &argc, argv, NULL, NULL) != napi_ok) return 0;
+  if (argc < want) { napi_throw_type_error(env, NULL, "missing arguments"); return 0; }
+  return 1;
+}
+
+static int get_int(napi_env env, napi_value v, int *out) {
+  if (napi_get_value_int32(env, v, out) != napi_ok) { napi_throw_type_error(env, NULL, "expected a number"); return 0; }
+  return 1;
+}
+
+static int get_double(napi_env env, napi_value v, double *out) {
+  if (napi_get_value_double(env, v, out) != napi_ok) { napi_throw_type_error(env, NULL, "expected a number"); return 0; }
+  return 1;
+}
+
+static int prop(napi_env env, napi_value obj, const char *key, napi_value *out) {
+  if (napi_get_named_property(env, obj, key, out) != napi_ok) { throw_napi(env, key); return 0; }
+  return 1;
+}
+
+static int prop_int(napi_env env, napi_value obj, const char *key, int *out) {
+  napi_value v;
+  return prop(env, obj, key, &v) && get_int(env, v, out);
+}
+
+static int prop_double(napi_env env, napi_value obj, const char *key, double *out) {
+  napi_value v;
+  return prop(env, obj, key, &v) && get_double(env, v, out);
+}
+
+static FdtdHipContext *handle(napi_env env, napi_value v) {
+  void *p = NULL;
+  if (napi_get_value_external(env, v, &p) != napi_ok || !p) {
+    napi_throw_type_error(env, NULL, "expected a live problem handle");
+    return NULL;
+  }
+  return (FdtdHipContext *)p;
+}
+
+static napi_value DeviceCount(napi_env env, napi_callback_info info) {
+  (void)info;
+  int n = 0;
+  if (fdtd_hip_device_count(&n) != FDTD_HIP_OK) n = 0;
+  napi_value r;
+  CALL(env, napi_create_int32(env, n, &r));
+  return r;
+}
+
+static napi_value Init(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  if (!get_args(env, info, 1, argv)) return NULL;
+  FdtdHipDims d;
+  memset(&d, 0, sizeof d);
+  if (!prop_int(env, argv[0], "nx", &d.nx) || !prop_int(env, argv[0], "ny", &d.ny) ||
+      !prop_int(env, argv[0], "nz", &d.nz) || !prop_double(env, argv[0], "dx", &d.dx) ||
+      !prop_double(env, argv[0], "dy", &d.dy) || !prop_double(env, argv[0], "dz", &d.dz) ||
+      !prop_double(env, argv[0], "dt", &d.dt) ||
+      !prop_int(env, argv[0], "numberOfTimeSteps", &d.numberOfTimeSteps) ||
+      !prop_int(env, argv[0], "precision", &d.precision) || !prop_int(env, argv[0], "boundary", &d.boundary))
+    return NULL;
+  FdtdHipContext *ctx = NULL;
+  if (fdtd_hip_init(&d, &ctx) != FDTD_HIP_OK) return throw_lib(env, "init");
+  napi_value ext;
+  CALL(env, napi_create_external(env, ctx, NULL, NULL, &ext));
+  return ext;
+}
+
+static napi_value UploadCoeff(napi_env env, napi_callback_info info) {
+  napi_value argv[3];
+  int which, ok;
+  size_t n;
+  if (!get_args(env, info, 3, argv)) return NULL;
+  FdtdHipContext *c = handle(env, argv[0]);
+  if (!c || !get_int(env, argv[1], &which)) return NULL;
+  double *p = typed(env, argv[2], napi_float64_array, &n, 0, &ok);
+  if (!ok) return NULL;
+  if (fdtd_hip_upload_coeff(c, which, p, (long long)n) != FDTD_HIP_OK) return throw_lib(env, "uploadCoeff");
+  return undefined(env);
+}
+
+static napi_value UploadCpml(napi_env env, napi_callback_info info) {
+  napi_value argv[3];
+  int face, nc, ms, es, ok;
+  if (!get_args(env, info, 3, argv)) return NULL;
+  FdtdHipContext *c = handle(env, argv[0]);
+  if (!c || !get_int(env, argv[1], &face)) return NULL;
+  napi_value o = argv[2], v;
+  if (!prop_int(env, o, "nc", &nc) || !prop_int(env, o, "m_start", &ms) || !prop_int(env, o, "e_start", &es)) return NULL;
+  const char *pk[4] = {"b_e", "a_e", "b_m", "a_m"};
+  double *prof[4];
+  for (int q = 0; q < 4; ++q) {
+    if (!prop(env, o, pk[q], &v)) return NULL;
+    prof[q] = typed(env, v, napi_float64_array, NULL, 0, &ok);
+    if (!ok) return NULL;
+  }
+  const char *ck[4] = {"CPsi0", "CPsi1", "CPsi2", "CPsi3"};
+  double *cp[4];
+  size_t cn[4];
+  for (int q = 0; q < 4; ++q) {
+    if (!prop(env, o, ck[q], &v)) return NULL;
+    cp[q] = typed(env, v, napi_float64_array, &cn[q], 0, &ok);
+    if (!ok) return NULL;
+  }
+  if (fdtd_hip_upload_cpml(c, face, nc, prof[0], prof[1], prof[2], prof[3], cp[0], (long long)cn[0], cp[1],
+                           (long long)cn[1], cp[2], (long long)cn[2], cp[3], (long long)cn[3], ms, es) != FDTD_HIP_OK)
+    return throw_lib(env, "uploadCpml");
+  return undefined(env);
+}
+
+static napi_value UploadVSource(napi_env env, napi_callback_info info) {
+  napi_value argv[5];
+  int dir, ok;
+  size_t ni, nc, nw;
+  if (!get_args(env, info, 5, argv)) return NULL;
+  FdtdHipContext *c = handle(env, argv[0]);
+  if (!c || !get_int(env, argv[1], &dir)) return NULL;
+  int *idx = typed(env, argv[2], napi_int32_array, &ni, 0, &ok);
+  if (!ok) return NULL;
+  double *coef = typed(env, argv[3], napi_float64_array, &nc, 0, &ok);
+  if (!ok) return NULL;
+  double *wave = typed(env, argv[4], napi_float64_array, &nw, 0, &ok);
+  if (!ok) return NULL;
+  if (ni != nc) { napi_throw_range_error(env, NULL, "uploadVSource: idx and coef lengths differ"); return NULL; }
+  if (fdtd_hip_upload_vsource(c, dir, idx, coef, (int)ni, wave, (int)nw) != FDTD_HIP_OK) return throw_lib(env, "uploadVSource");
+  return undefined(env);
+}
+
+static napi_value UploadSVoltage(napi_env env, napi_callback_info info) {
+  napi_value argv[4];
+  int dir, ok;
+  size_t n;
+  double csvf;
+  if (!get_args(env, info, 4, argv)) return NULL;
+  FdtdHipContext *c = handle(env, argv[0]);
+  if (!c || !get_int(env, argv[1], &dir)) return NULL;
+  int *idx = typed(env, argv[2], napi_int32_array, &n, 0, &ok);
+  if (!ok || !get_double(env, argv[3], &csvf)) return NULL;
+  if (fdtd_hip_upload_svoltage(c, dir, idx, (int)n, csvf) != FDTD_HIP_OK) return throw_lib(env, "uploadSVoltage");
+  return undefined(env);
+}
+
+static napi_value UploadSCurrent(napi_env env, napi_callback_info info) {
+  napi_value argv[5];
+  int neg, ok;
+  size_t n1, n2, n3;
+  if (!get_args(env, info, 5, argv)) return NULL;
+  FdtdHipContext *c = handle(env, argv[0]);
+  if (!c) return NULL;
+  int *comp = typed(env, argv[1], napi_int32_array, &n1, 0, &ok);
+  if (!ok) return NULL;
+  int *index = typed(env, argv[2], napi_int32_array, &n2, 0, &ok);
+  if (!ok) return NULL;
+  double *w = typed(env, argv[3], napi_float64_array, &n3, 0, &ok);
+  if (!ok || !get_int(env, argv[4], &neg)) return NULL;
+  if (n1 != n2 || n1 != n3) { napi_throw_range_error(env, NULL, "uploadSCurrent: term lists differ in length"); return NULL; }
+  if (fdtd_hip_upload_scurrent(c, comp, index, w, (int)n1, neg) != FDTD_HIP_OK) return throw_lib(env, "uploadSCurrent");
+  return undefined(env);
+}
+
+static napi_value UploadFarfield(napi_env env, napi_callback_info info) {
+  napi_value argv[4];
+  int nf, ok;
+  size_t nb, np;
+  if (!get_args(env, info, 4, argv)) return NULL;
+  FdtdHipContext *c = handle(env, argv[0]);
+  if (!c || !get_int(env, argv[1], &nf)) return NULL;
+  int *box = typed(env, argv[2], napi_int32_array, &nb, 0, &ok);
+  if (!ok) return NULL;
+  double *ph = typed(env, argv[3], napi_float64_array, &np, 0, &ok);
+  if (!ok) return NULL;
+  if (nb != 6 || nf <= 0) { napi_throw_range_error(env, NULL, "uploadFarfield: box needs 6 entries"); return NULL; }
+  if (fdtd_hip_upload_farfield(c, nf, box, ph, (int)(np / (4 * (size_t)nf))) != FDTD_HIP_OK) return throw_lib(env, "uploadFarfield");
+  return undefined(env);
+}
+
+static napi_value UploadPortDft(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  int ok;
+  size_t n;
+  if (!get_args(env, info, 2, argv)) return NULL;
+  FdtdHipContext *c = handle(env, argv[0]);
+  if (!c) return NULL;
+  double *f = typed(env, argv[1], napi_float64_array, &n, 0, &ok);
+  if (!ok) return NULL;
+  if (fdtd_hip_upload_port_dft(c, (int)n, f) != FDTD_HIP_OK) return throw_lib(env, "uploadPortDft");
+  return undefined(env);
+}
+
+static napi_value RunBatch(napi_env env, napi_callback_info info) {
+  napi_value argv[5];
+  int ts, count, ok;
+  if (!get_args(env, info, 5, argv)) return NULL;
+  FdtdHipContext *c = handle(env, argv[0]);
+  if (!c || !get_int(env, argv[1], &ts) || !get_int(env, argv[2], &count)) return NULL;
+  double *sv = typed(env, argv[3], napi_float64_array, NULL, 1, &ok);
+  if (!ok) return NULL;
+  double *si = typed(env, argv[4], napi_float64_array, NULL, 1, &ok);
+  if (!ok) return NULL;
+  if (fdtd_hip_run_batch(c, ts, count, sv, si) != FDTD_HIP_OK) return throw_lib(env, "runBatch");
+  return undefined(env);
+}
+
+static napi_value ReadField(napi_env env, napi_callback_info info) {
+  napi_value argv[3];
+  int which, ok;
+  size_t n;
+  if (!get_args(env, info, 3, argv)) return NULL;
+  FdtdHipContext *c = handle(env, argv[0]);
+  if (!c || !get_int(env, argv[1], &which)) return NULL;
+  double *p = typed(env, argv[2], napi_float64_array, &n, 0, &ok);
+  if (!ok) return NULL;
+  if (fdtd_hip_read_field(c, which, p, (long long)n) != FDTD_HIP_OK) return throw_lib(env, "readField");
+  return undefined(env);
+}
+
+static napi_value ReadFarfield(napi_env env, napi_callback_info info) {
+  napi_value argv[4];
+  int which, ok;
+  size_t n1, n2;
+  if (!get_args(env, info, 4, argv)) return NULL;
+  FdtdHipContext *c = handle(env, argv[0]);
+  if (!c || !get_int(env, argv[1], &which)) return NULL;
+  double *re = typed(env, argv[2], napi_float64_array, &n1, 0, &ok);
+  if (!ok) return NULL;
+  double *im = typed(env, argv[3], napi_float64_array, &n2, 0, &ok);
+  if (!ok) return NULL;
+  if (n1 != n2) { napi_throw_range_error(env, NULL, "readFarfield: re/im lengths differ"); return NULL; }
+  if (fdtd_hip_read_farfield(c, which, re, im, (long long)n1) != FDTD_HIP_OK) return throw_lib(env, "readFarfield");
+  return undefined(env);
+}
+
+static napi_value ReadPortDft(napi_env env, napi_callback_info info) {
+  napi_value argv[3];
+  int ok;
+  size_t n1, n2;
+  if (!get_args(env, info, 3, argv)) return NULL;
+  FdtdHipContext *c = handle(env, argv[0]);
+  if (!c) return NULL;
+  double *re = typed(env, argv[1], napi_float64_array, &n1, 0, &ok);
+  if (!ok) return NULL;
+  double *im = typed(env, argv[2], napi_float64_array, &n2, 0, &ok);
+  if (!ok) return NULL;
+  if (n1 != n2) { napi_throw_range_error(env, NULL, "readPortDft: re/im lengths differ"); return NULL; }
+  if (fdtd_hip_read_port_dft(c, re, im, (long long)n1) != FDTD_HIP_OK) return throw_lib(env, "readPortDft");
+  return undefined(env);
+}
+
+static napi_value SetTiming(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  bool on = false;
+  if (!get_args(env, info, 2, argv)) return NULL;
+  FdtdHipContext *c = handle(env, argv[0]);
+  if (!c) return NULL;
+  napi_get_value_bool(env, argv[1], &on);
+  if (fdtd_hip_set_timing(c, on ? 1 : 0) != FDTD_HIP_OK) return throw_lib(env, "setTiming");
+  return undefined(env);
+}
+
+static napi_value KernelTimes(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  if (!get_args(env, info, 1, argv)) return NULL;
+  FdtdHipContext *c = handle(env, argv[0]);
+  if (!c) return NULL;
+  FdtdHipKernelTimes t;
+  if (fdtd_hip_get_kernel_times(c, &t) != FDTD_HIP_OK) return throw_lib(env, "kernelTimes");
+  napi_value obj, ms, la, x;
+  CALL(env, napi_create_object(env, &obj));
+  CALL(env, napi_create_array_with_length(env, 4, &ms));
+  CALL(env, napi_create_array_with_length(env, 4, &la));
+  for (uint32_t q = 0; q < 4; ++q) {
+    CALL(env, napi_create_double(env, t.ms[q], &x));
+    CALL(env, napi_set_element(env, ms, q, x));
+    CALL(env, napi_create_double(env, (double)t.launches[q], &x));
+    CALL(env, napi_set_element(env, la, q, x));
+  }
+  CALL(env, napi_set_named_property(env, obj, "ms", ms));
+  CALL(env, napi_set_named_property(env, obj, "launches", la));
+  return obj;
+}
+
+static napi_value Destroy(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  if (!get_args(env, info, 1, argv)) return NULL;
+  FdtdHipContext *c = handle(env, argv[0]);
+  if (!c) return NULL;
+  fdtd_hip_destroy(c);
+  return undefined(env);
+}
+
+static napi_value Register(napi_env env, napi_value exports) {
+  napi_property_descriptor d[] = {
+    {"deviceCount", NULL, DeviceCount, NULL, NULL, NULL, napi_enumerable, NULL},
+    {"init", NULL, Init, NULL, NULL, NULL, napi_enumerable, NULL},
+    {"uploadCoeff", NULL, UploadCoeff, NULL, NULL, NULL, napi_enumerable, NULL},
+    {"uploadCpml", NULL, UploadCpml, NULL, NULL, NULL, napi_enumerable, NULL},
+    {"uploadVSource", NULL, UploadVSource, NULL, NULL, NULL, napi_enumerable, NULL},
+    {"uploadSVoltage", NULL, UploadSVoltage, NULL, NULL, NULL, napi_enumerable, NULL},
+    {"uploadSCurrent", NULL, UploadSCurrent, NULL, NULL, NULL, napi_enumerable, NULL},
+    {"uploadFarfield", NULL, UploadFarfield, NULL, NULL, NULL, napi_enumerable, NULL},
+    {"uploadPortDft", NULL, UploadPortDft, NULL, NULL, NULL, napi_enumerable, NULL},
+    {"runBatch", NULL, RunBatch, NULL, NULL, NULL, napi_enumerable, NULL},
+    {"readField", NULL, ReadField, NULL, NULL, NULL, napi_enumerable, NULL},
+    {"readFarfield", NULL, ReadFarfield, NULL, NULL, NULL, napi_enumerable, NULL},
+    {"readPortDft", NULL, ReadPortDft, NULL, NULL, NULL, napi_enumerable, NULL},
+    {"setTiming", NULL, SetTiming, NULL, NULL, NULL, napi_enumerable, NULL},
+    {"kernelTimes", NULL, KernelTimes, NULL, NULL, NULL, napi_enumerable, NULL},
+    {"destroy", NULL, Destroy, NULL, NULL, NULL, napi_enumerable, NULL},
+  };
+  if (napi_define_properties(env, exports, sizeof d / sizeof d[0], d) != napi_ok) {
+    throw_napi(env, "define_properties");
+    return NULL;
+  }
+  return exports;
+}
+
+NAPI_MODULE(NODE_GYP_MODULE_NAME, Register)

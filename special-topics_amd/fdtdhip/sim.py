@@ -1,0 +1,175 @@
+"""Python host of the HIP time loop: uploads a problem file through the C ABI and runs it.
+
+``HipSim`` mirrors what the Node backend (js/backends/hipBackend.mjs) does with a
+js_fdtd problem object; the test-suite and bench.py use it to drive the library from
+the same problem files the JavaScript builder writes.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import native
+from .problem import COEFF_ORDER, FACE_NAMES, FIELD_ORDER, Problem
+
+
+def _d(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _f(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def _i(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_int))
+
+
+SLOT_COMP = {"x": ["Hy", "Hz", "Ey", "Ez"], "y": ["Hx", "Hz", "Ez", "Ex"], "z": ["Hx", "Hy", "Ex", "Ey"]}
+
+
+class HipSim:
+    """One problem resident on the GPU."""
+
+    def __init__(self, problem: Problem, precision: str = "fp64", boundary: str = "reference",
+                 port_dft_freqs: np.ndarray | None = None, farfield: bool = True):
+        self.L = native.lib()
+        self.problem = problem
+        g = problem.grid
+        self.fp32 = precision == "fp32"
+        dims = native.FdtdHipDims(
+            nx=g["nx"], ny=g["ny"], nz=g["nz"], dx=g["dx"], dy=g["dy"], dz=g["dz"], dt=g["dt"],
+            numberOfTimeSteps=problem.steps, precision=native.FP32 if self.fp32 else native.FP64,
+            boundary=native.BOUNDARY_REFERENCE if boundary == "reference" else native.BOUNDARY_MATLAB)
+        h = C.c_void_p()
+        native.check(self.L.fdtd_hip_init(C.byref(dims), C.byref(h)), "init")
+        self.h = h
+        try:
+            self._upload(problem, port_dft_freqs, farfield)
+        except Exception:
+            self.close()
+            raise
+
+    def _upload(self, P: Problem, port_dft_freqs, farfield: bool):
+        L, h = self.L, self.h
+        for w, name in enumerate(COEFF_ORDER):
+            a = P.arrays["coeff/" + name]
+            if a.dtype == np.float32:
+                if not self.fp32:
+                    raise ValueError("FP32 problem file cannot drive an FP64 run")
+                a = np.ascontiguousarray(a)
+                native.check(L.fdtd_hip_upload_coeff_f32(h, w, _f(a), a.size), f"upload_coeff {name}")
+            else:
+                a = np.ascontiguousarray(a, dtype=np.float64)
+                native.check(L.fdtd_hip_upload_coeff(h, w, _d(a), a.size), f"upload_coeff {name}")
+        for f in P.header["cpml"]:
+            face = f["face"]
+            prof = [np.ascontiguousarray(P.arrays[f"cpml/{face}/{p}"], dtype=np.float64)
+                    for p in ("b_e", "a_e", "b_m", "a_m")]
+            cps = [np.ascontiguousarray(P.arrays[f"cpml/{face}/CPsi{s}"], dtype=np.float64)
+                   for s in range(4)]
+            args = []
+            for a in cps:
+                args += [_d(a), a.size]
+            native.check(L.fdtd_hip_upload_cpml(h, FACE_NAMES.index(face), f["nc"], *[_d(a) for a in prof],
+                                                *args, f["m_start"], f["e_start"]), f"upload_cpml {face}")
+        for n, vs in enumerate(P.header["vsources"]):
+            idx = np.ascontiguousarray(P.arrays[f"vsrc/{n}/idx"], dtype=np.int32)
+            coef = np.ascontiguousarray(P.arrays[f"vsrc/{n}/coef"], dtype=np.float64)
+            vpe = np.ascontiguousarray(P.arrays[f"vsrc/{n}/vpe"], dtype=np.float64)
+            native.check(L.fdtd_hip_upload_vsource(h, vs["dir"], _i(idx), _d(coef), idx.size, _d(vpe), vpe.size),
+                         "upload_vsource")
+        for n, sv in enumerate(P.header["svoltages"]):
+            idx = np.ascontiguousarray(P.arrays[f"sv/{n}/idx"], dtype=np.int32)
+            native.check(L.fdtd_hip_upload_svoltage(h, sv["dir"], _i(idx), idx.size, float(sv["Csvf"])),
+                         "upload_svoltage")
+        for n, si in enumerate(P.header["scurrents"]):
+            comp = np.ascontiguousarray(P.arrays[f"si/{n}/comp"], dtype=np.int32)
+            index = np.ascontiguousarray(P.arrays[f"si/{n}/index"], dtype=np.int32)
+            w = np.ascontiguousarray(P.arrays[f"si/{n}/weight"], dtype=np.float64)
+            native.check(L.fdtd_hip_upload_scurrent(h, _i(comp), _i(index), _d(w), comp.size, int(si["negate"])),
+                         "upload_scurrent")
+        ff = P.header["ff"]
+        self.nfreq = 0
+        if farfield and ff.get("nFreq", 0) > 0 and "ff/phase" in P.arrays:
+            box = np.array([ff["li"], ff["lj"], ff["lk"], ff["ui"], ff["uj"], ff["uk"]], dtype=np.int32)
+            ph = np.ascontiguousarray(P.arrays["ff/phase"], dtype=np.float64)
+            native.check(L.fdtd_hip_upload_farfield(h, ff["nFreq"], _i(box), _d(ph), P.steps), "upload_farfield")
+            self.nfreq = ff["nFreq"]
+            self.ff_sizes = ([ff["nj_ff"] * ff["nk_ff"]] * 8 + [ff["ni_ff"] * ff["nk_ff"]] * 8 +
+                             [ff["ni_ff"] * ff["nj_ff"]] * 8)
+        self.n_sv = len(P.header["svoltages"])
+        self.n_si = len(P.header["scurrents"])
+        self.ndft = 0
+        if port_dft_freqs is not None and len(port_dft_freqs) > 0:
+            fr = np.ascontiguousarray(port_dft_freqs, dtype=np.float64)
+            native.check(L.fdtd_hip_upload_port_dft(h, fr.size, _d(fr)), "upload_port_dft")
+            self.ndft = fr.size
+
+    # ------------------------------------------------------------------------------
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.fdtd_hip_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def run(self, ts0: int, count: int):
+        """Steps [ts0, ts0+count); returns (V[n_sv, count], I[n_si, count])."""
+        v = np.zeros((max(self.n_sv, 1), count))
+        i = np.zeros((max(self.n_si, 1), count))
+        native.check(self.L.fdtd_hip_run_batch(self.h, ts0, count, _d(v), _d(i)), "run_batch")
+        return v[: self.n_sv], i[: self.n_si]
+
+    def run_no_readback(self, ts0: int, count: int):
+        native.check(self.L.fdtd_hip_run_batch(self.h, ts0, count, None, None), "run_batch")
+
+    def synchronize(self):
+        native.check(self.L.fdtd_hip_synchronize(self.h), "synchronize")
+
+    def field(self, comp: str) -> np.ndarray:
+        out = np.zeros(self.problem.component_len(comp))
+        native.check(self.L.fdtd_hip_read_field(self.h, FIELD_ORDER.index(comp), _d(out), out.size), "read_field")
+        return out
+
+    def set_field(self, comp: str, values: np.ndarray):
+        a = np.ascontiguousarray(values, dtype=np.float64)
+        native.check(self.L.fdtd_hip_write_field(self.h, FIELD_ORDER.index(comp), _d(a), a.size), "write_field")
+
+    def psi_len(self, face: str, slot: int) -> int:
+        f = next(f for f in self.problem.header["cpml"] if f["face"] == face)
+        d = list(self.problem.component_dims(SLOT_COMP[face[0]][slot]))
+        d["xyz".index(face[0])] = f["nc"]
+        return d[0] * d[1] * d[2]
+
+    def psi(self, face: str, slot: int) -> np.ndarray:
+        out = np.zeros(self.psi_len(face, slot))
+        native.check(self.L.fdtd_hip_read_psi(self.h, FACE_NAMES.index(face), slot, _d(out), out.size), "read_psi")
+        return out
+
+    def set_psi(self, face: str, slot: int, values: np.ndarray):
+        a = np.ascontiguousarray(values, dtype=np.float64)
+        native.check(self.L.fdtd_hip_write_psi(self.h, FACE_NAMES.index(face), slot, _d(a), a.size), "write_psi")
+
+    def farfield(self, which: int):
+        n = self.ff_sizes[which] * self.nfreq
+        re, im = np.zeros(n), np.zeros(n)
+        native.check(self.L.fdtd_hip_read_farfield(self.h, which, _d(re), _d(im), n), "read_farfield")
+        return re, im
+
+    def port_dft(self):
+        n = (self.n_sv + self.n_si) * self.ndft
+        re, im = np.zeros(n), np.zeros(n)
+        native.check(self.L.fdtd_hip_read_port_dft(self.h, _d(re), _d(im), n), "read_port_dft")
+        return (re + 1j * im).reshape(self.n_sv + self.n_si, self.ndft)
+
+    def set_timing(self, enable: bool):
+        native.check(self.L.fdtd_hip_set_timing(self.h, 1 if enable else 0), "set_timing")
+
+    def kernel_times(self):
+        t = native.FdtdHipKernelTimes()
+        native.check(self.L.fdtd_hip_get_kernel_times(self.h, C.byref(t)), "get_kernel_times")
+        return list(t.ms), list(t.launches)

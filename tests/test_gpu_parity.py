@@ -1,0 +1,147 @@
+"""HIP time loop vs the reference golden vectors and the CPU oracle (needs the GPU).
+
+FP64 results must be bit-identical: the kernels evaluate the reference's expression
+trees without contraction.  FP32 results are compared against FP64 within stated
+tolerances.
+"""
+
+import numpy as np
+import pytest
+
+import goldens as G
+from oracle import OracleSim
+
+pytestmark = pytest.mark.gpu
+
+
+def _sim(problem, **kw):
+    from fdtdhip.sim import HipSim
+    return HipSim(problem, **kw)
+
+
+def _check_run_golden(sim, fx, fields=True):
+    steps = fx["steps"]
+    v, i = sim.run(0, steps)
+    for n, b in enumerate(fx["v"]):
+        assert np.array_equal(v[n], G.f64(b)), f"V{n} differs from the reference trace"
+    for n, b in enumerate(fx["i"]):
+        assert np.array_equal(i[n], G.f64(b)), f"I{n} differs from the reference trace"
+    if fields:
+        for comp in ("Hx", "Hy", "Hz", "Ex", "Ey", "Ez"):
+            assert G.sha(sim.field(comp)) == fx["fields"][comp], f"{comp} differs"
+        for key, h in fx["fields"].items():
+            if key.startswith("psi/"):
+                _, face, nm = key.split("/")
+                slot = G.PSI_NAMES[face[0]].index(nm)
+                assert G.sha(sim.psi(face, slot)) == h, f"Psi {face}/{nm} differs"
+    if fx.get("ff"):
+        for n, name in enumerate(G.FF_ORDER):
+            re, im = sim.farfield(n)
+            assert G.sha(re) == fx["ff"][name]["re"] and G.sha(im) == fx["ff"][name]["im"], f"NTFF {name} differs"
+    return v, i
+
+
+def test_golden_small_fp64_matches_reference_run(get_problem):
+    prob = get_problem("golden-small")
+    fx = G.load("run-golden-small.json")
+    sim = _sim(prob)
+    _check_run_golden(sim, fx)
+    sim.close()
+
+
+def test_random_state_one_step_fp64_matches_oracle(get_problem):
+    """Seeded random fields and CPML state exercise every coefficient and Psi slot."""
+    prob = get_problem("golden-small")
+    fields, psi = G.random_state(prob, 7)
+    sim = _sim(prob)
+    ref = OracleSim(prob)
+    for comp, a in fields.items():
+        sim.set_field(comp, a)
+        ref.field(comp)[:] = a
+    for (face, slot), a in psi.items():
+        sim.set_psi(face, slot, a)
+        ref.psi(face, slot)[:] = a
+    for ts in (0, 1, 2):
+        v, i = sim.run(ts, 1)
+        rv, ri = ref.run(ts, 1)
+        assert np.array_equal(v[:, 0], rv[0]) and np.array_equal(i[:, 0], ri[0])
+        for comp in fields:
+            assert np.array_equal(sim.field(comp), ref.field(comp)), f"step {ts}: {comp}"
+        for face, slot in psi:
+            assert np.array_equal(sim.psi(face, slot), ref.psi(face, slot)), f"step {ts}: psi {face}/{slot}"
+    sim.close()
+
+
+def test_matlab_boundary_fp64_matches_oracle(get_problem):
+    prob = get_problem("golden-small")
+    sim = _sim(prob, boundary="matlab")
+    ref = OracleSim(prob, boundary="matlab")
+    v, i = sim.run(0, 120)
+    rv, ri = ref.run(0, 120)
+    assert np.array_equal(v, rv.T) and np.array_equal(i, ri.T)
+    for comp in ("Hx", "Hy", "Hz", "Ex", "Ey", "Ez"):
+        assert np.array_equal(sim.field(comp), ref.field(comp)), comp
+    sim.close()
+
+
+def test_batches_compose(get_problem):
+    """Running in uneven batches gives the same trace as one batch."""
+    prob = get_problem("golden-small")
+    a = _sim(prob)
+    va, ia = a.run(0, 100)
+    a.close()
+    b = _sim(prob)
+    parts = [b.run(s, n) for s, n in ((0, 1), (1, 37), (38, 50), (88, 12))]
+    b.close()
+    assert np.array_equal(va, np.concatenate([p[0] for p in parts], axis=1))
+    assert np.array_equal(ia, np.concatenate([p[1] for p in parts], axis=1))
+
+
+@pytest.mark.parametrize("scenario", ["ifa-matlab-7000", "ifa-config1-7000"])
+def test_ifa_fp64_full_run_matches_reference(get_problem, scenario):
+    """The 39x136x136 IFA, 7000 FP64 steps, against the reference's own per-step code."""
+    prob = get_problem(scenario)
+    fx = G.load(f"run-{scenario}.json")
+    sim = _sim(prob)
+    _check_run_golden(sim, fx)
+    sim.close()
+
+
+def test_port_running_dft_matches_trace_dft(get_problem):
+    prob = get_problem("golden-small")
+    freqs = np.arange(1, 41) * 0.5e9
+    sim = _sim(prob, port_dft_freqs=freqs)
+    v, i = sim.run(0, prob.steps)
+    dft = sim.port_dft()
+    sim.close()
+    dt = prob.grid["dt"]
+    n = np.arange(prob.steps)
+    w = 2 * np.pi * freqs
+    for o, (x, shift) in enumerate([(v[0], 0.0), (v[1], 0.0), (i[0], -dt / 2), (i[1], -dt / 2)]):
+        t = (n + 1) * dt + shift
+        ref = dt * (np.cos(np.outer(w, t)) @ x - 1j * (np.sin(np.outer(w, t)) @ x))
+        scale = np.max(np.abs(ref))
+        assert np.max(np.abs(dft[o] - ref)) <= 1e-12 * scale
+
+
+def test_fp32_tracks_fp64(get_problem):
+    prob = get_problem("golden-small")
+    s64 = _sim(prob)
+    v64, i64 = s64.run(0, prob.steps)
+    s64.close()
+    s32 = _sim(prob, precision="fp32")
+    v32, i32 = s32.run(0, prob.steps)
+    s32.close()
+    for a, b in ((v32, v64), (i32, i64)):
+        err = np.max(np.abs(a - b)) / np.max(np.abs(b))
+        assert err < 2e-4, err
+
+
+def test_kernel_timing_counts_launches(get_problem):
+    prob = get_problem("golden-small")
+    sim = _sim(prob)
+    sim.set_timing(True)
+    sim.run(0, 5)
+    ms, launches = sim.kernel_times()
+    sim.close()
+    assert launches[0] == 5 and launches[1] == 5 and ms[0] > 0 and ms[1] > 0
