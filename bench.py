@@ -1,0 +1,209 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X Yee FDTD time loop.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload roofline-256]
+
+One "step" is one FDTD time step (H update, E update, port epilogue, NTFF when the
+scenario has one) over the whole grid.  The default workload is the 256^3 FP32
+stencil-roofline scenario (seeded random dielectric boxes, 8-cell CPML, centre
+dipole; special-topics_amd/scenarios/roofline-256.json).  Prints one JSON line with
+Mcell-updates/s (whole job), the HBM-roofline position of the dominant kernel
+(timed with HIP events on the library's stream), and a CPU baseline: the oracle's
+reference-structured loop (x-slab threads for the bulk, serial CPML/captures, FP64)
+timed on this host for a bounded number of steps of the same scenario.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import tempfile
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(REPO, "special-topics_amd"))
+
+HBM_PEAK_GBS = 8000.0          # MI355X HBM3E
+FP32_BYTES_PER_CELL_PHASE = 72  # 3 own-field reads + 3 writes + 3 neighbour-field reads + 9 coefficients, x4 B
+PSI_BYTES = 12                  # read Psi, write Psi, read CPsi (FP32)
+
+
+def psi_counts(problem):
+    """(H-phase Psi elements, E-phase Psi elements) of the problem's CPML faces."""
+    slot_comp = {"x": ["Hy", "Hz", "Ey", "Ez"], "y": ["Hx", "Hz", "Ez", "Ex"], "z": ["Hx", "Hy", "Ex", "Ey"]}
+    nh = ne = 0
+    for f in problem.header["cpml"]:
+        axis = "xyz".index(f["face"][0])
+        for slot in range(4):
+            d = list(problem.component_dims(slot_comp[f["face"][0]][slot]))
+            d[axis] = f["nc"]
+            n = d[0] * d[1] * d[2]
+            if slot < 2:
+                nh += n
+            else:
+                ne += n
+    return nh, ne
+
+
+def cpu_baseline(problem64, steps: int, threads: int, target_s: float = 15.0):
+    """Oracle steps/s on the host; steps = 0 sizes the sample for about target_s."""
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import oracle  # the checker, used here only as the CPU baseline
+    sim = oracle.OracleSim(problem64, threads=threads)
+    sim.run(0, 1)                      # first touch of all pages
+    ts = 1
+    if steps <= 0:
+        t0 = time.perf_counter()
+        sim.run(ts, 2)
+        ts += 2
+        per = (time.perf_counter() - t0) / 2
+        steps = int(max(2, min(problem64.steps - ts, (target_s - 2 * per) / per)))
+    t0 = time.perf_counter()
+    sim.run(ts, steps)
+    dt = time.perf_counter() - t0
+    nx, ny, nz = problem64.dims
+    del sim
+    return nx * ny * nz * steps / dt / 1e6, dt, steps
+
+
+def host_cores() -> int:
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=int(os.environ.get("WORLD_SIZE", "1")))
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--workload", default="roofline-256")
+    ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64"])
+    ap.add_argument("--timing-steps", type=int, default=100)
+    ap.add_argument("--cpu-steps", type=int, default=0, help="0 = choose for ~20 s of CPU work")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--workdir", default=os.path.join(tempfile.gettempdir(), "fdtd-bench"))
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="first run a plain device copy of this many MiB (calibrates rocprof byte counters)")
+    ap.add_argument("--planes", type=int, default=-1, help="x planes per workgroup (-1 = autotune)")
+    ap.add_argument("--cells", type=int, default=0, help="cells per thread (with --planes)")
+    ap.add_argument("--coef", type=int, default=0, help="coefficient storage (with --planes)")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
+              "(the time loop is single-GPU; see DESIGN.md for how it would shard).", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from fdtdhip.problem import load_scenario_problem
+    from fdtdhip.sim import HipSim
+
+    os.makedirs(args.workdir, exist_ok=True)
+    if args.copy_calibration_mib:
+        n = args.copy_calibration_mib * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+    fp32 = args.precision == "fp32"
+    total_steps = args.warmup + args.steps
+    t_setup = time.perf_counter()
+    problem = load_scenario_problem(args.workload, args.workdir, steps=max(total_steps, args.timing_steps + 1),
+                                    fp32=fp32, no_phase=False)
+    nx, ny, nz = problem.dims
+    ncells = nx * ny * nz
+    sim = HipSim(problem, precision=args.precision)
+    if args.planes < 0:
+        tuning = sim.autotune(3)
+    else:
+        sim.set_tuning(args.planes, args.cells, args.coef)
+        tuning = sim.tuning()
+    setup_s = time.perf_counter() - t_setup
+
+    # Throughput: W untimed steps, then exactly K steps between device-wide syncs.
+    sim.run_no_readback(0, args.warmup)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    sim.run_no_readback(args.warmup, args.steps)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    sim.close()
+
+    # Per-kernel durations with HIP events on the library stream (fresh state).
+    sim = HipSim(problem, precision=args.precision)
+    sim.set_tuning(*tuning)
+    sim.set_timing(True)
+    sim.run_no_readback(0, args.timing_steps)
+    ms, launches = sim.kernel_times()
+    sim.close()
+    avg_ms = [m / max(n, 1) for m, n in zip(ms, launches)]
+    nh, ne = psi_counts(problem)
+    esz = 4 if fp32 else 8
+    # Algorithmic bytes: the bulk kernels move 18 values per cell (3 own-field reads and
+    # writes, 3 neighbour-field reads, 9 coefficients); the CPML kernels 3 values per
+    # Psi element (Psi read and write, CPsi read).
+    bulk_bytes = FP32_BYTES_PER_CELL_PHASE * ncells * esz // 4
+    bytes_h = bulk_bytes + PSI_BYTES * nh * esz // 4
+    bytes_e = bulk_bytes + PSI_BYTES * ne * esz // 4
+    dom = 0 if avg_ms[0] >= avg_ms[1] else 1
+    dom_bytes = bulk_bytes
+    achieved = dom_bytes / (avg_ms[dom] * 1e-3) / 1e9
+
+    mcells = ncells * args.steps / elapsed / 1e6
+    line = {
+        "metric": "Mcell-updates/s",
+        "value": round(mcells, 1),
+        "unit": "Mcell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(elapsed / args.steps * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if fp32 else "f64",
+        "data": "synthetic",
+        "config": {"workload": f"{args.workload} {nx}x{ny}x{nz} {args.precision.upper()} fields+coefficients, "
+                               "FP64 port DFT/traces", "cells": ncells, "precision": args.precision,
+                   "parallelism": "single GPU"},
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "k_update_h" if dom == 0 else "k_update_e",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": None,
+            "algorithmic_bytes_per_launch": dom_bytes,
+            "avg_launch_ms": round(avg_ms[dom], 5),
+        },
+        "kernels_avg_ms": {"update_h": round(avg_ms[0], 5), "cpml_h": round(avg_ms[4], 5),
+                           "update_e": round(avg_ms[1], 5), "cpml_e": round(avg_ms[5], 5),
+                           "port_epilogue": round(avg_ms[2], 5), "ntff": round(avg_ms[3], 5)},
+        "hbm_equivalent_frac_step": round((bytes_h + bytes_e) / (elapsed / args.steps) / 1e9 / HBM_PEAK_GBS, 4),
+        "setup_s": round(setup_s, 2),
+        "tuning": {"planes_per_workgroup": tuning[0], "cells_per_thread": tuning[1],
+                   "coefficient_storage": ["indexed", "arrays"][tuning[2]]},
+    }
+
+    if not args.no_cpu_baseline:
+        threads = max(1, min(host_cores() - 1, 15))   # the box gives one GPU a 16-core share
+        p64 = load_scenario_problem(args.workload, args.workdir, steps=max(total_steps, args.timing_steps + 1),
+                                    fp32=False, no_phase=False)
+        mc, secs, steps_cpu = cpu_baseline(p64, args.cpu_steps, threads)
+        line["cpu_baseline"] = {
+            "value": round(mc, 2), "unit": "Mcell-updates/s", "cores": threads, "kind": "port",
+            "sample": f"{steps_cpu} FP64 steps of {args.workload} ({secs:.1f} s): x-slab threads for the bulk "
+                      "curls, CPML/captures on one thread, as the reference's wasm-cpu engine is structured",
+        }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

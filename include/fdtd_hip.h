@@ -56,8 +56,9 @@ typedef struct FdtdHipContext FdtdHipContext;
 
 /* Accumulated device time per kernel kind while timing is enabled. */
 typedef struct {
-    double ms[4];        /* 0 = H update, 1 = E update, 2 = port epilogue, 3 = NTFF */
-    long long launches[4];
+    double ms[6];        /* 0 = H bulk, 1 = E bulk, 2 = port epilogue, 3 = NTFF,
+                            4 = H CPML terms, 5 = E CPML terms */
+    long long launches[6];
 } FdtdHipKernelTimes;
 
 /* Last error text of the calling thread.  Replaces fdtd_last_error (fdtd_cuda.h:22). */
@@ -149,6 +150,30 @@ int fdtd_hip_read_farfield(FdtdHipContext *ctx, int which, double *re, double *i
 /* Copy the running port DFTs: [nSampledVoltages + nSampledCurrents][nFreq] complex,
  * voltages first, already multiplied by dt. */
 int fdtd_hip_read_port_dft(FdtdHipContext *ctx, double *re, double *im, long long n);
+
+/* Launch shape and coefficient storage of the stencil kernels.  Every setting gives
+ * bit-identical results; only speed and memory differ.  Zero fields select the
+ * defaults. */
+typedef struct {
+    int planes_per_workgroup;  /* x planes a workgroup marches through */
+    int cells_per_thread;      /* consecutive cells per thread: 1, 2 or (FP32) 4 */
+    int coefficient_storage;   /* 0: deduplicated (self, curl, curl) triples with 8- or
+                                  16-bit per-cell indices when they fit, else per-cell
+                                  arrays; 1: per-cell arrays */
+} FdtdHipTuning;
+
+int fdtd_hip_set_tuning(FdtdHipContext *ctx, const FdtdHipTuning *tuning);
+int fdtd_hip_get_tuning(FdtdHipContext *ctx, FdtdHipTuning *tuning);
+
+/* Autotuner: time every launch shape and coefficient storage for a few steps of this
+ * problem, keep the fastest, then zero the fields, CPML state, accumulators and traces
+ * again.  Only before the first fdtd_hip_run_batch.  `chosen` may be NULL. */
+int fdtd_hip_autotune(FdtdHipContext *ctx, int steps_per_candidate, FdtdHipTuning *chosen);
+
+/* Storage in use for the uploaded coefficients (0 = per-cell arrays, 1 = 8-bit
+ * indices, 2 = 16-bit indices) and, if entries != NULL, the number of distinct
+ * triples of Hx,Hy,Hz,Ex,Ey,Ez (0 when not deduplicated). */
+int fdtd_hip_get_coefficient_storage(FdtdHipContext *ctx, int *mode, int *entries);
 
 /* Per-kernel device timing with HIP events on the library's stream. */
 int fdtd_hip_set_timing(FdtdHipContext *ctx, int enable);

@@ -8,11 +8,13 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 using namespace fdtd;
@@ -85,12 +87,24 @@ struct FdtdHipContext {
   int ndft = 0;
   double *dft_w = nullptr, *dft_acc = nullptr;
 
+  FdtdHipTuning tune{0, 0, 0};  // requested launch shape / coefficient storage (0 = default)
+  int coef_mode = -1;           // CoefMode in use; -1 = not decided yet
+  int coef_built = 0;           // deduplicated tables built (idx width in coef_idx_bytes)
+  int coef_idx_bytes = 0;       // 0 = triples do not fit 16 bits
+  long long steps_run = 0;
+  std::vector<double> host_coef[NCOEFF];   // uniform-layout copies kept for deduplication
+  void *coef_idx = nullptr;     // 6 index arrays (uint8 or uint16), uniform layout with guards
+  void *coef_tab[6] = {};
+  int coef_n[6] = {};
   int timing = 0;
   hipEvent_t ev[8] = {};
   FdtdHipKernelTimes times{};
+  CpmlBoxes cpml_h{}, cpml_e{};
 
-  void *field_ptr(int c) const { return (char *)fields + c * stride_bytes; }
-  void *coeff_ptr(int w) const { return (char *)coeffs + w * stride_bytes; }
+  // Plane 0 of array c / w (one guard plane into its allocation slot).
+  void *field_ptr(int c) const { return (char *)fields + c * stride_bytes + (size_t)SI * esz; }
+  void *coeff_ptr(int w) const { return (char *)coeffs + w * stride_bytes + (size_t)SI * esz; }
+  size_t idx_stride(int esz_idx) const { return ((size_t)(d.nx + 3) * SI * esz_idx + 255) / 256 * 256; }
   void comp_dims(int c, int out[3]) const {
     out[0] = d.nx + kExt[c][0];
     out[1] = d.ny + kExt[c][1];
@@ -122,6 +136,12 @@ Grid<T> make_grid(const FdtdHipContext *c) {
   g.Hx = (T *)c->field_ptr(0); g.Hy = (T *)c->field_ptr(1); g.Hz = (T *)c->field_ptr(2);
   g.Ex = (T *)c->field_ptr(3); g.Ey = (T *)c->field_ptr(4); g.Ez = (T *)c->field_ptr(5);
   for (int w = 0; w < NCOEFF; ++w) g.C[w] = (const T *)c->coeff_ptr(w);
+  const int isz = c->coef_idx_bytes ? c->coef_idx_bytes : 1;
+  for (int q = 0; q < 6; ++q) {
+    g.ct.idx[q] = c->coef_idx ? (const char *)c->coef_idx + q * c->idx_stride(isz) + (size_t)c->SI * isz : nullptr;
+    g.ct.tab[q] = c->coef_tab[q];
+    g.ct.n[q] = c->coef_n[q];
+  }
   return g;
 }
 
@@ -186,14 +206,21 @@ int upload_coeff_impl(FdtdHipContext *c, int which, const S *host, long long n) 
   const int comp = kCoeffOwner[which];
   if (n != c->comp_len(comp))
     return fail("upload_coeff: array %d has %lld elements, expected %lld", which, n, c->comp_len(comp));
+  pack_uniform(c, comp, host, c->host_coef[which]);
+  c->coef_mode = -1;
+  if (c->coef_built) {                       // tables no longer match the arrays
+    for (auto &t : c->coef_tab) { (void)hipFree(t); t = nullptr; }
+    (void)hipFree(c->coef_idx);
+    c->coef_idx = nullptr;
+    c->coef_built = 0;
+    c->coef_idx_bytes = 0;
+    for (int &n : c->coef_n) n = 0;
+  }
   if (c->fp32()) {
-    std::vector<float> v;
-    pack_uniform(c, comp, host, v);
+    std::vector<float> v(c->host_coef[which].begin(), c->host_coef[which].end());
     return upload_vec(c, c->coeff_ptr(which), v);
   }
-  std::vector<double> v;
-  pack_uniform(c, comp, host, v);
-  return upload_vec(c, c->coeff_ptr(which), v);
+  return upload_vec(c, c->coeff_ptr(which), c->host_coef[which]);
 }
 
 template <typename T>
@@ -218,40 +245,217 @@ int ensure_traces(FdtdHipContext *c) {
   return FDTD_HIP_OK;
 }
 
+// Coefficient ids of each component's (self, curl1, curl2) triple, Hx..Ez.
+const int kTriple[6][3] = {{CHXH, CHXEY, CHXEZ}, {CHYH, CHYEZ, CHYEX}, {CHZH, CHZEX, CHZEY},
+                           {CEXE, CEXHZ, CEXHY}, {CEYE, CEYHX, CEYHZ}, {CEZE, CEZHY, CEZHX}};
+
+struct TripleKey {
+  uint64_t a, b, c;
+  bool operator==(const TripleKey &o) const { return a == o.a && b == o.b && c == o.c; }
+};
+struct TripleHash {
+  size_t operator()(const TripleKey &k) const {
+    return (size_t)(k.a * 0x9E3779B97F4A7C15ULL ^ (k.b + 0x632BE59BD9B4E019ULL) * 0xC2B2AE3D27D4EB4FULL ^ k.c);
+  }
+};
+
+// Deduplicate each component's coefficient triples over its Yee shape (table order =
+// first occurrence) once, into 8-bit indices when every component has at most 256
+// distinct triples and 16-bit ones up to 65536; then pick the storage the tuning asks
+// for (default: the indices when they exist, else the per-cell arrays).
+int build_coefficient_tables(FdtdHipContext *c) {
+  if (c->coef_built) return FDTD_HIP_OK;
+  for (int w = 0; w < NCOEFF; ++w)
+    if (c->host_coef[w].empty()) return fail("run_batch: coefficient array %d was never uploaded", w);
+  std::vector<std::vector<uint32_t>> index(6);
+  std::vector<std::vector<double>> table(6);
+  size_t maxn = 0;
+  for (int q = 0; q < 6 && maxn <= 65536; ++q) {
+    int t[3];
+    c->comp_dims(q, t);
+    std::unordered_map<TripleKey, uint32_t, TripleHash> seen;
+    index[q].assign(c->ntot, 0);
+    const double *a0 = c->host_coef[kTriple[q][0]].data(), *a1 = c->host_coef[kTriple[q][1]].data(),
+                 *a2 = c->host_coef[kTriple[q][2]].data();
+    for (long long i = 0; i < t[0] && seen.size() <= 65536; ++i)
+      for (long long j = 0; j < t[1]; ++j)
+        for (long long k = 0; k < t[2]; ++k) {
+          const long long o = i * c->SI + j * c->SJ + k;
+          TripleKey key;
+          std::memcpy(&key.a, a0 + o, 8); std::memcpy(&key.b, a1 + o, 8); std::memcpy(&key.c, a2 + o, 8);
+          auto it = seen.find(key);
+          uint32_t id;
+          if (it == seen.end()) {
+            id = (uint32_t)seen.size();
+            seen.emplace(key, id);
+            table[q].push_back(a0[o]); table[q].push_back(a1[o]); table[q].push_back(a2[o]);
+          } else {
+            id = it->second;
+          }
+          index[q][o] = id;
+        }
+    maxn = std::max(maxn, seen.size());
+  }
+  c->coef_built = 1;
+  if (maxn > 65536) { c->coef_idx_bytes = 0; return FDTD_HIP_OK; }
+  const int isz = maxn <= (size_t)U8_TABLE ? 1 : 2;
+  const size_t stride = c->idx_stride(isz);
+  if (dev_alloc(c, &c->coef_idx, 6 * stride)) return FDTD_HIP_ERR;
+  for (int q = 0; q < 6; ++q) {
+    void *dst = (char *)c->coef_idx + q * stride + (size_t)c->SI * isz;
+    if (isz == 1) {
+      std::vector<uint8_t> v(index[q].begin(), index[q].end());
+      if (upload_vec(c, dst, v)) return FDTD_HIP_ERR;
+    } else {
+      std::vector<uint16_t> v(index[q].begin(), index[q].end());
+      if (upload_vec(c, dst, v)) return FDTD_HIP_ERR;
+    }
+    c->coef_n[q] = (int)(table[q].size() / 3);
+    if (c->fp32()) {
+      std::vector<float> v(table[q].begin(), table[q].end());
+      if (dev_alloc(c, &c->coef_tab[q], v.size() * 4) || upload_vec(c, c->coef_tab[q], v)) return FDTD_HIP_ERR;
+    } else {
+      if (dev_alloc(c, &c->coef_tab[q], table[q].size() * 8) || upload_vec(c, c->coef_tab[q], table[q])) return FDTD_HIP_ERR;
+    }
+  }
+  c->coef_idx_bytes = isz;
+  return FDTD_HIP_OK;
+}
+
+int finalize_coefficients(FdtdHipContext *c) {
+  if (c->coef_mode >= 0) return FDTD_HIP_OK;
+  if (c->tune.coefficient_storage == 1) { c->coef_mode = COEF_ARRAYS; return FDTD_HIP_OK; }
+  if (build_coefficient_tables(c)) return FDTD_HIP_ERR;
+  c->coef_mode = c->coef_idx_bytes == 1 ? COEF_U8 : (c->coef_idx_bytes == 2 ? COEF_U16 : COEF_ARRAYS);
+  return FDTD_HIP_OK;
+}
+
+// Tiles of 256 plane offsets x `chunk` x-planes for `planes` planes.
+int cells_per_thread(const FdtdHipContext *c) {
+  const int w = c->tune.cells_per_thread;
+  return w > 0 ? w : 1;
+}
+
+MarchGeom march_geom(const FdtdHipContext *c, int planes) {
+  MarchGeom m;
+  m.ncol = c->SI / (256 * cells_per_thread(c));
+  m.chunk = c->tune.planes_per_workgroup > 0 ? c->tune.planes_per_workgroup : 2;
+  m.nchunk = (planes + m.chunk - 1) / m.chunk;
+  m.per_xcd = (m.ncol * m.nchunk + 7) / 8;
+  return m;
+}
+
+template <typename T, int CM>
+void launch_h_w(FdtdHipContext *c, const Grid<T> &g, const MarchGeom &m) {
+  const dim3 grid(8 * m.per_xcd), block(256);
+  switch (cells_per_thread(c)) {
+    case 4: hipLaunchKernelGGL((k_update_h<T, CM, 4>), grid, block, 0, c->stream, g, m); break;
+    case 2: hipLaunchKernelGGL((k_update_h<T, CM, 2>), grid, block, 0, c->stream, g, m); break;
+    default: hipLaunchKernelGGL((k_update_h<T, CM, 1>), grid, block, 0, c->stream, g, m); break;
+  }
+}
+
+template <typename T, int CM>
+void launch_e_w(FdtdHipContext *c, const Grid<T> &g, const MarchGeom &m, int ghost) {
+  const dim3 grid(8 * m.per_xcd), block(256);
+  switch (cells_per_thread(c)) {
+    case 4: hipLaunchKernelGGL((k_update_e<T, CM, 4>), grid, block, 0, c->stream, g, m, ghost); break;
+    case 2: hipLaunchKernelGGL((k_update_e<T, CM, 2>), grid, block, 0, c->stream, g, m, ghost); break;
+    default: hipLaunchKernelGGL((k_update_e<T, CM, 1>), grid, block, 0, c->stream, g, m, ghost); break;
+  }
+}
+
+template <typename T>
+void launch_h(FdtdHipContext *c, const Grid<T> &g, const MarchGeom &m) {
+  if (c->coef_mode == COEF_U8) launch_h_w<T, COEF_U8>(c, g, m);
+  else if (c->coef_mode == COEF_U16) launch_h_w<T, COEF_U16>(c, g, m);
+  else launch_h_w<T, COEF_ARRAYS>(c, g, m);
+}
+
+template <typename T>
+void launch_e(FdtdHipContext *c, const Grid<T> &g, const MarchGeom &m, int ghost) {
+  if (c->coef_mode == COEF_U8) launch_e_w<T, COEF_U8>(c, g, m, ghost);
+  else if (c->coef_mode == COEF_U16) launch_e_w<T, COEF_U16>(c, g, m, ghost);
+  else launch_e_w<T, COEF_ARRAYS>(c, g, m, ghost);
+}
+
+// Disjoint boxes covering every cell a CPML face of the phase (0 = H, 1 = E) acts on.
+CpmlBoxes cpml_boxes(const FdtdHipContext *c, int phase) {
+  const int n[3] = {c->d.nx, c->d.ny, c->d.nz};
+  std::vector<std::pair<int, int>> in[3], out[3];
+  for (int a = 0; a < 3; ++a) {
+    for (int s = 0; s < 2; ++s) {
+      const auto &f = c->face[2 * a + s];
+      if (!f.nc) continue;
+      const int lo = phase == 0 ? f.m_start : f.e_start;
+      in[a].push_back({lo, lo + f.nc});
+    }
+    std::sort(in[a].begin(), in[a].end());
+    int pos = 0;
+    for (auto &iv : in[a]) {
+      if (iv.first > pos) out[a].push_back({pos, iv.first});
+      pos = std::max(pos, iv.second);
+    }
+    if (pos < n[a] + 1) out[a].push_back({pos, n[a] + 1});
+  }
+  CpmlBoxes bx{};
+  auto add = [&](std::pair<int, int> x, std::pair<int, int> y, std::pair<int, int> z) {
+    if (x.second <= x.first || y.second <= y.first || z.second <= z.first || bx.nbox >= MAX_CPML_BOXES) return;
+    const int b = bx.nbox++;
+    bx.lo[b][0] = x.first; bx.lo[b][1] = y.first; bx.lo[b][2] = z.first;
+    bx.ext[b][0] = x.second - x.first; bx.ext[b][1] = y.second - y.first; bx.ext[b][2] = z.second - z.first;
+    bx.start[b + 1] = bx.start[b] + (long long)bx.ext[b][0] * bx.ext[b][1] * bx.ext[b][2];
+  };
+  const std::pair<int, int> fy{0, n[1] + 1}, fz{0, n[2] + 1};
+  for (auto &x : in[0]) add(x, fy, fz);
+  for (auto &x : out[0]) for (auto &y : in[1]) add(x, y, fz);
+  for (auto &x : out[0]) for (auto &y : out[1]) for (auto &z : in[2]) add(x, y, z);
+  return bx;
+}
+
 template <typename T>
 int launch_step(FdtdHipContext *c, int ts) {
   const Grid<T> g = make_grid<T>(c);
   const FaceSet<T> fs = make_faces<T>(c);
   const dim3 block(256);
   const int ghost = c->d.boundary == FDTD_HIP_BOUNDARY_REFERENCE;
-  const dim3 gh((c->SI + 255) / 256, c->d.nx + 1);
-  const dim3 ge((c->SI + 255) / 256, ghost ? c->d.nx : c->d.nx + 1);
+  const MarchGeom mh = march_geom(c, c->d.nx + 1), me = march_geom(c, ghost ? c->d.nx : c->d.nx + 1);
   const bool t = c->timing;
 
   if (t) HIP_TRY(hipEventRecord(c->ev[0], c->stream));
-  hipLaunchKernelGGL(k_update_h<T>, gh, block, 0, c->stream, g, fs);
+  launch_h<T>(c, g, mh);
   if (t) HIP_TRY(hipEventRecord(c->ev[1], c->stream));
-  hipLaunchKernelGGL(k_update_e<T>, ge, block, 0, c->stream, g, fs, ghost);
+  if (c->cpml_h.nbox)
+    hipLaunchKernelGGL(k_cpml_h<T>, dim3((unsigned)((c->cpml_h.start[c->cpml_h.nbox] + 255) / 256)), block, 0,
+                       c->stream, g, fs, c->cpml_h);
   if (t) HIP_TRY(hipEventRecord(c->ev[2], c->stream));
+  launch_e<T>(c, g, me, ghost);
+  if (t) HIP_TRY(hipEventRecord(c->ev[3], c->stream));
+  if (c->cpml_e.nbox)
+    hipLaunchKernelGGL(k_cpml_e<T>, dim3((unsigned)((c->cpml_e.start[c->cpml_e.nbox] + 255) / 256)), block, 0,
+                       c->stream, g, fs, c->cpml_e);
+  if (t) HIP_TRY(hipEventRecord(c->ev[4], c->stream));
   if (c->ports.nvs + c->ports.nsv + c->ports.nsi > 0) {
     PortState ps = c->ports;
     ps.ndft = c->ndft; ps.w = c->dft_w; ps.dft = c->dft_acc;
     hipLaunchKernelGGL(k_port_epilogue<T>, dim3(1), block, 0, c->stream, g, ps, ts);
   }
-  if (t) HIP_TRY(hipEventRecord(c->ev[3], c->stream));
+  if (t) HIP_TRY(hipEventRecord(c->ev[5], c->stream));
   if (c->nff > 0) {
     const long total = 2 * ((long)c->ff.nj * c->ff.nk + (long)c->ff.ni * c->ff.nk + (long)c->ff.ni * c->ff.nj);
     hipLaunchKernelGGL(k_ntff_direct<T>, dim3((unsigned)((total + 255) / 256)), block, 0, c->stream, g, c->ff, ts);
   }
-  if (t) HIP_TRY(hipEventRecord(c->ev[4], c->stream));
+  if (t) HIP_TRY(hipEventRecord(c->ev[6], c->stream));
   HIP_TRY(hipGetLastError());
   if (t) {
-    HIP_TRY(hipEventSynchronize(c->ev[4]));
-    for (int q = 0; q < 4; ++q) {
+    // events 0..6 bracket: H bulk, H CPML, E bulk, E CPML, port epilogue, NTFF
+    static const int slot[6] = {0, 4, 1, 5, 2, 3};
+    HIP_TRY(hipEventSynchronize(c->ev[6]));
+    for (int q = 0; q < 6; ++q) {
       float ms = 0.f;
       HIP_TRY(hipEventElapsedTime(&ms, c->ev[q], c->ev[q + 1]));
-      c->times.ms[q] += ms;
-      c->times.launches[q] += 1;
+      c->times.ms[slot[q]] += ms;
+      c->times.launches[slot[q]] += 1;
     }
   }
   return FDTD_HIP_OK;
@@ -288,18 +492,19 @@ int fdtd_hip_init(const FdtdHipDims *dims, FdtdHipContext **out) {
   if (d.boundary != FDTD_HIP_BOUNDARY_REFERENCE && d.boundary != FDTD_HIP_BOUNDARY_MATLAB)
     return fail("init: unknown boundary mode %d", d.boundary);
   if (d.numberOfTimeSteps < 1) return fail("init: numberOfTimeSteps must be positive");
-  const long long ntot = (long long)(d.nx + 1) * (d.ny + 1) * (d.nz + 1);
+  const long long ntot = (long long)(d.nx + 3) * ((long long)(d.ny + 1) * (d.nz + 4) + 1023);
   if (ntot >= (1LL << 31) - 1) return fail("init: %lld cells exceed the 32-bit device indexing", ntot);
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail("init: no HIP device");
 
   auto *c = new FdtdHipContext();
   c->d = d;
-  c->SJ = d.nz + 1;
-  c->SI = (d.ny + 1) * (d.nz + 1);
-  c->ntot = ntot;
+  c->SJ = (d.nz + 1 + 3) / 4 * 4;
+  c->SI = ((d.ny + 1) * c->SJ + 1023) / 1024 * 1024;
+  c->ntot = (long long)(d.nx + 1) * c->SI;
   c->esz = c->fp32() ? 4 : 8;
-  c->stride_bytes = ((size_t)ntot * c->esz + 255) / 256 * 256;
+  // Each uniform array: guard plane, nx+1 planes, guard plane.
+  c->stride_bytes = ((size_t)(d.nx + 3) * c->SI * c->esz + 255) / 256 * 256;
   if (hipGetDevice(&c->device) != hipSuccess) c->device = 0;
   int rc = FDTD_HIP_OK;
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) rc = fail("init: stream creation failed");
@@ -497,11 +702,14 @@ int fdtd_hip_run_batch(FdtdHipContext *c, int tsStart, int count, double *svOut,
   if (tsStart < 0 || count < 0 || tsStart + count > c->d.numberOfTimeSteps)
     return fail("run_batch: steps [%d,%d) outside [0,%d)", tsStart, tsStart + count, c->d.numberOfTimeSteps);
   if (c->ndft && c->ports.nsv + c->ports.nsi == 0) return fail("run_batch: port DFT without observers");
-  if (ensure_traces(c)) return FDTD_HIP_ERR;
+  if (ensure_traces(c) || finalize_coefficients(c)) return FDTD_HIP_ERR;
+  c->cpml_h = cpml_boxes(c, 0);
+  c->cpml_e = cpml_boxes(c, 1);
   for (int s = 0; s < count; ++s) {
     const int rc = c->fp32() ? launch_step<float>(c, tsStart + s) : launch_step<double>(c, tsStart + s);
     if (rc) return rc;
   }
+  c->steps_run += count;
   for (int o = 0; svOut && o < c->ports.nsv; ++o)
     HIP_TRY(hipMemcpyAsync(svOut + (size_t)o * count, c->ports.vtr + (size_t)o * c->ports.steps + tsStart,
                            count * sizeof(double), hipMemcpyDeviceToHost, c->stream));
@@ -599,6 +807,105 @@ int fdtd_hip_read_port_dft(FdtdHipContext *c, double *re, double *im, long long 
   return FDTD_HIP_OK;
 }
 
+static int check_tuning(const FdtdHipContext *c, const FdtdHipTuning *t, const char *who) {
+  if (t->planes_per_workgroup < 0 || t->planes_per_workgroup > 4096)
+    return fail("%s: %d planes per workgroup out of range", who, t->planes_per_workgroup);
+  const int maxw = c->fp32() ? 4 : 2;
+  if (t->cells_per_thread != 0 && t->cells_per_thread != 1 && t->cells_per_thread != 2 &&
+      !(t->cells_per_thread == 4 && maxw == 4))
+    return fail("%s: %d cells per thread unsupported (1, 2%s)", who, t->cells_per_thread, maxw == 4 ? " or 4" : "");
+  if (t->coefficient_storage < 0 || t->coefficient_storage > 1)
+    return fail("%s: coefficient storage %d unknown", who, t->coefficient_storage);
+  return FDTD_HIP_OK;
+}
+
+int fdtd_hip_set_tuning(FdtdHipContext *c, const FdtdHipTuning *t) {
+  if (!c || !t) return fail("set_tuning: null argument");
+  if (check_tuning(c, t, "set_tuning")) return FDTD_HIP_ERR;
+  if (t->coefficient_storage != c->tune.coefficient_storage) c->coef_mode = -1;
+  c->tune = *t;
+  return FDTD_HIP_OK;
+}
+
+int fdtd_hip_get_tuning(FdtdHipContext *c, FdtdHipTuning *t) {
+  if (!c || !t) return fail("get_tuning: null argument");
+  *t = c->tune;
+  return FDTD_HIP_OK;
+}
+
+// Zero the dynamic state (fields, CPML Psi, NTFF and port-DFT accumulators, traces).
+static int reset_state(FdtdHipContext *c) {
+  HIP_TRY(hipMemsetAsync(c->fields, 0, 6 * c->stride_bytes, c->stream));
+  for (auto &f : c->face)
+    for (int s = 0; s < 4; ++s)
+      if (f.psi[s]) HIP_TRY(hipMemsetAsync(f.psi[s], 0, f.n[s] * c->esz, c->stream));
+  if (c->ff_buf) {
+    long long total = 0;
+    for (int a = 0; a < 24; ++a) total += 2 * c->ff_sizes[a];
+    HIP_TRY(hipMemsetAsync(c->ff_buf, 0, total * sizeof(double), c->stream));
+  }
+  if (c->dft_acc) HIP_TRY(hipMemsetAsync(c->dft_acc, 0, 2 * sizeof(double) * std::max(1, c->ports.nsv + c->ports.nsi) * c->ndft, c->stream));
+  if (c->traces_ready) {
+    HIP_TRY(hipMemsetAsync(c->ports.vtr, 0, sizeof(double) * std::max(1, c->ports.nsv) * c->ports.steps, c->stream));
+    HIP_TRY(hipMemsetAsync(c->ports.itr, 0, sizeof(double) * std::max(1, c->ports.nsi) * c->ports.steps, c->stream));
+  }
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return FDTD_HIP_OK;
+}
+
+int fdtd_hip_autotune(FdtdHipContext *c, int steps_per_candidate, FdtdHipTuning *chosen) {
+  if (!c) return fail("autotune: null context");
+  if (c->steps_run) return fail("autotune: must run before the first time step (it resets the state)");
+  if (steps_per_candidate < 1) steps_per_candidate = 2;
+  const int steps = std::min(steps_per_candidate + 1, c->d.numberOfTimeSteps);
+  if (ensure_traces(c) || build_coefficient_tables(c)) return FDTD_HIP_ERR;
+  c->cpml_h = cpml_boxes(c, 0);
+  c->cpml_e = cpml_boxes(c, 1);
+  const FdtdHipTuning saved = c->tune;
+  const int saved_timing = c->timing;
+  std::vector<FdtdHipTuning> cands;
+  const int ws[3] = {1, 2, 4};
+  const int planes[4] = {1, 2, 4, 8};
+  for (int cs = 0; cs < 2; ++cs) {
+    if (cs == 0 && !c->coef_idx_bytes) continue;
+    for (int w : ws) {
+      if (w == 4 && !c->fp32()) continue;
+      for (int pl : planes) cands.push_back(FdtdHipTuning{pl, w, cs});
+    }
+  }
+  double best = 1e300;
+  FdtdHipTuning best_t = saved;
+  for (const auto &t : cands) {
+    c->tune = t;
+    c->coef_mode = -1;
+    if (finalize_coefficients(c)) return FDTD_HIP_ERR;
+    c->timing = 1;
+    c->times = FdtdHipKernelTimes{};
+    for (int s = 0; s < steps; ++s) {
+      if (s == 1) c->times = FdtdHipKernelTimes{};     // first step warms up
+      const int rc = c->fp32() ? launch_step<float>(c, s) : launch_step<double>(c, s);
+      if (rc) return rc;
+    }
+    const double ms = c->times.ms[0] + c->times.ms[1] + c->times.ms[4] + c->times.ms[5];
+    if (ms < best) { best = ms; best_t = t; }
+  }
+  c->tune = best_t;
+  c->coef_mode = -1;
+  c->timing = saved_timing;
+  c->times = FdtdHipKernelTimes{};
+  if (finalize_coefficients(c) || reset_state(c)) return FDTD_HIP_ERR;
+  if (chosen) *chosen = best_t;
+  return FDTD_HIP_OK;
+}
+
+int fdtd_hip_get_coefficient_storage(FdtdHipContext *c, int *mode, int *entries) {
+  if (!c || !mode) return fail("get_coefficient_storage: null argument");
+  if (finalize_coefficients(c)) return FDTD_HIP_ERR;
+  *mode = c->coef_mode;
+  if (entries) for (int q = 0; q < 6; ++q) entries[q] = c->coef_n[q];
+  return FDTD_HIP_OK;
+}
+
 int fdtd_hip_set_timing(FdtdHipContext *c, int enable) {
   if (!c) return fail("set_timing: null context");
   c->timing = enable ? 1 : 0;
@@ -625,6 +932,8 @@ void fdtd_hip_destroy(FdtdHipContext *c) {
     for (int s = 0; s < 4; ++s) { (void)hipFree(f.psi[s]); (void)hipFree(f.cpsi[s]); }
   }
   for (void *p : c->small_allocs) (void)hipFree(p);
+  for (void *p : c->coef_tab) (void)hipFree(p);
+  (void)hipFree(c->coef_idx);
   (void)hipFree(c->ff_buf);
   (void)hipFree(c->fields);
   (void)hipFree(c->coeffs);

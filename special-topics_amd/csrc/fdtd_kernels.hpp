@@ -2,9 +2,10 @@
 //
 // Device layout.  All six field components and all 18 coefficient arrays share one
 // "uniform" shape (nx+1) x (ny+1) x (nz+1), k fastest: element (i,j,k) of every
-// component sits at i*SI + j*SJ + k with SJ = nz+1, SI = (ny+1)*SJ.  Each component
-// occupies the sub-box of its own Yee shape; the other slots stay zero and are never
-// written.  One flat offset therefore addresses Hx..Ez and their coefficients at the
+// component sits at i*SI + j*SJ + k with SJ = nz+1 and SI = (ny+1)*SJ rounded up to a
+// multiple of 64 elements (so every 256-wide tile starts on a 128-byte line).  Each
+// component occupies the sub-box of its own Yee shape; the other slots stay zero and
+// are never written.  One zero guard plane precedes plane 0 and one follows plane nx.  One flat offset therefore addresses Hx..Ez and their coefficients at the
 // same cell, the curl neighbours are at +-1, +-SJ, +-SI for every component, and the
 // plane i = 0 of Hz / Ex coincides with the reference's flat layout of those arrays
 // (both have rows of nz+1), which is what the ghost-plane reads need.
@@ -25,16 +26,118 @@ enum CoeffId {
   CHXH, CHXEZ, CHXEY, CHYH, CHYEX, CHYEZ, CHZH, CHZEY, CHZEX, NCOEFF
 };
 
+// Deduplicated coefficients: per component (Hx,Hy,Hz,Ex,Ey,Ez) an index array in the
+// uniform layout and a table of n (self, curl1, curl2) triples holding the exact
+// values of the per-cell arrays.  Real grids have a few hundred distinct triples at
+// most (materials x interface averages x CPML grading), so the per-cell coefficient
+// traffic drops from three values to one 1- or 2-byte index.
+struct CoefTables {
+  const void *idx[6];
+  const void *tab[6];
+  int n[6];
+};
+
+// Coefficient storage modes (kernel template argument).
+enum CoefMode { COEF_ARRAYS = 0, COEF_U8 = 1, COEF_U16 = 2 };
+constexpr int U8_TABLE = 256;
+
 template <typename T>
 struct Grid {
   int nx, ny, nz;
   int SJ, SI;
   T *Hx, *Hy, *Hz, *Ex, *Ey, *Ez;
   const T *C[NCOEFF];
+  CoefTables ct;
 };
 
-// One CPML face.  Psi/CPsi keep the reference layout: the component's shape with the
-// extent along the face axis replaced by nc.
+// LDS copy of the three tables a kernel uses (COEF_U8 only).
+template <typename T, int CM>
+struct TableCache {
+  T v[CM == COEF_U8 ? 3 * 3 * U8_TABLE : 1];
+};
+
+template <typename T, int CM>
+__device__ __forceinline__ void load_tables(const Grid<T> &g, int first, TableCache<T, CM> &tc) {
+  if (CM != COEF_U8) return;
+  for (int q = threadIdx.x; q < 3 * 3 * U8_TABLE; q += blockDim.x) {
+    const int comp = q / (3 * U8_TABLE), r = q - comp * (3 * U8_TABLE);
+    tc.v[q] = r < 3 * g.ct.n[first + comp] ? ((const T *)g.ct.tab[first + comp])[r] : T(0);
+  }
+  __syncthreads();
+}
+
+// W consecutive cells of one row, loaded and stored as one W*sizeof(T)-byte access.
+template <typename T, int W>
+struct alignas(sizeof(T) * W) Vec {
+  T v[W];
+  __device__ __forceinline__ T &operator[](int e) { return v[e]; }
+  __device__ __forceinline__ const T &operator[](int e) const { return v[e]; }
+};
+
+template <typename T, int W>
+__device__ __forceinline__ Vec<T, W> ldv(const T *__restrict__ p) { return *(const Vec<T, W> *)p; }
+template <typename T, int W>
+__device__ __forceinline__ void stv(T *__restrict__ p, const Vec<T, W> &v) { *(Vec<T, W> *)p = v; }
+
+template <typename T, int W>
+__device__ __forceinline__ Vec<T, W> ldv_stream(const T *__restrict__ p) {
+  Vec<T, W> r;
+  if (W * sizeof(T) == 16) {
+    typedef float f4 __attribute__((ext_vector_type(4)));
+    const f4 x = __builtin_nontemporal_load((const f4 *)p);
+    __builtin_memcpy(&r, &x, 16);
+  } else if (W * sizeof(T) == 8) {
+    typedef float f2 __attribute__((ext_vector_type(2)));
+    const f2 x = __builtin_nontemporal_load((const f2 *)p);
+    __builtin_memcpy(&r, &x, 8);
+  } else {
+#pragma unroll
+    for (int e = 0; e < W; ++e) r[e] = __builtin_nontemporal_load(p + e);
+  }
+  return r;
+}
+
+// The (self, curl1, curl2) coefficients of component `first + local` for the W cells
+// starting at offset c.
+template <typename T, int CM, int W>
+__device__ __forceinline__ void coeffs(const CoefTables &ct, const TableCache<T, CM> &tc, int first, int local,
+                                       int c, const T *__restrict__ a0, const T *__restrict__ a1,
+                                       const T *__restrict__ a2, Vec<T, W> &c0, Vec<T, W> &c1, Vec<T, W> &c2) {
+  if (CM == COEF_ARRAYS) {
+    c0 = ldv_stream<T, W>(a0 + c);
+    c1 = ldv_stream<T, W>(a1 + c);
+    c2 = ldv_stream<T, W>(a2 + c);
+    return;
+  }
+  int id[W];
+  if (CM == COEF_U8) {
+    const uint8_t *ip = (const uint8_t *)ct.idx[first + local] + c;
+    if (W == 4) {
+      const uint32_t x = __builtin_nontemporal_load((const uint32_t *)ip);
+#pragma unroll
+      for (int e = 0; e < W; ++e) id[e] = 3 * (int)((x >> (8 * e)) & 0xff);
+    } else if (W == 2) {
+      const uint16_t x = __builtin_nontemporal_load((const uint16_t *)ip);
+#pragma unroll
+      for (int e = 0; e < W; ++e) id[e] = 3 * (int)((x >> (8 * e)) & 0xff);
+    } else {
+#pragma unroll
+      for (int e = 0; e < W; ++e) id[e] = 3 * (int)__builtin_nontemporal_load(ip + e);
+    }
+  } else {
+    const uint16_t *ip = (const uint16_t *)ct.idx[first + local] + c;
+#pragma unroll
+    for (int e = 0; e < W; ++e) id[e] = 3 * (int)__builtin_nontemporal_load(ip + e);
+  }
+  const T *t = CM == COEF_U8 ? tc.v + local * 3 * U8_TABLE : (const T *)ct.tab[first + local];
+#pragma unroll
+  for (int e = 0; e < W; ++e) {
+    c0[e] = t[id[e]];
+    c1[e] = t[id[e] + 1];
+    c2[e] = t[id[e] + 2];
+  }
+}
+
 template <typename T>
 struct Face {
   int nc, m_start, e_start;
@@ -57,12 +160,232 @@ __device__ __forceinline__ int psi_offset(int i, int j, int k, int ci, int nc, i
   return (i * d1 + j) * nc + ci;
 }
 
-// Convolution term of the two faces (n then p) along AXIS for slot SLOT.  MAG selects
-// the magnetic profiles and H planes.  `diff` is the curl difference the face's Psi
-// integrates, identical to the one in the bulk update.
-template <typename T, int AXIS, int SLOT, bool MAG>
-__device__ __forceinline__ T cpml_term(const FaceSet<T> &fs, T val, T diff,
-                                       int i, int j, int k, int d1, int d2) {
+// Launch geometry shared by the two stencil kernels.  A thread updates W consecutive
+// cells of one row (rows are padded to a multiple of 4 elements, so the W cells never
+// straddle a row); a workgroup's 256 threads cover 256*W consecutive plane offsets
+// (a "column"; planes are padded to a multiple of 1024 elements, so every column
+// starts on a 128-byte line and all vector accesses, including the +-SJ and +-SI
+// neighbours, are aligned).  A tile is one column x `chunk` x-planes; the workgroup
+// marches through its planes keeping the x-neighbour values it needs next in
+// registers.  Tiles are numbered column-fastest and dealt to the 8 XCDs in contiguous
+// ranges (workgroup L runs on XCD L % 8), so the rows a tile reads from its
+// neighbour columns were fetched into the same XCD's L2 by the tile next to it.
+struct MarchGeom {
+  int ncol;       // columns per plane
+  int nchunk;     // x chunks
+  int chunk;      // planes per chunk
+  int per_xcd;    // tiles per XCD
+};
+
+__device__ __forceinline__ bool march_tile(const MarchGeom &m, int &col, int &ch) {
+  const int L = blockIdx.x;
+  const int tile = (L & 7) * m.per_xcd + (L >> 3);
+  if (tile >= m.ncol * m.nchunk) return false;
+  col = tile % m.ncol;
+  ch = tile / m.ncol;
+  return true;
+}
+
+// The field and coefficient arrays as no-alias pointers: the kernel bodies below take
+// them as __restrict__ parameters so the compiler may issue the next plane's loads
+// ahead of this plane's stores.
+template <typename T>
+struct Fields {
+  T *__restrict__ Hx; T *__restrict__ Hy; T *__restrict__ Hz;
+  T *__restrict__ Ex; T *__restrict__ Ey; T *__restrict__ Ez;
+};
+
+// H bulk update; W consecutive cells of a row per thread.  Hx over i in [0,nx], Hy/Hz
+// over i in [0,nx).  The CPML terms are added afterwards by k_cpml (the reference also
+// applies them after the whole bulk update, js_fdtd/src/fdtdWorker.js:62-79).  Every
+// array has a zero guard plane before plane 0 and after plane nx, so all neighbour
+// loads are unconditional; elements outside a component's Yee shape are written back
+// unchanged.
+template <typename T, int CM, int W>
+__device__ __forceinline__ void update_h_body(
+    T *__restrict__ Hx, T *__restrict__ Hy, T *__restrict__ Hz,
+    const T *__restrict__ Ex, const T *__restrict__ Ey, const T *__restrict__ Ez,
+    const Grid<T> &g, const MarchGeom &m, const TableCache<T, CM> &tc, int col, int ch) {
+#pragma clang fp contract(off)
+  typedef Vec<T, W> V;
+  const int p = (col * 256 + (int)threadIdx.x) * W;
+  const int nx = g.nx, ny = g.ny, nz = g.nz, SJ = g.SJ, SI = g.SI;
+  const int j = p / SJ;
+  const int k0 = p - j * SJ;
+  if (j > ny) return;                                      // plane padding
+  const int i0 = ch * m.chunk;
+  const int i1 = min(i0 + m.chunk, nx + 1);
+  int c = i0 * SI + p;
+  V ey_i = ldv<T, W>(Ey + c), ez_i = ldv<T, W>(Ez + c);   // E at plane i, carried forward
+  for (int i = i0; i < i1; ++i, c += SI) {
+    const bool inner = i < nx;
+    const V ey_n = ldv<T, W>(Ey + c + SI), ez_n = ldv<T, W>(Ez + c + SI);
+    const V ex_i = ldv<T, W>(Ex + c);
+    const V ez_j = ldv<T, W>(Ez + c + SJ), ex_j = ldv<T, W>(Ex + c + SJ);
+    const T ey_w = Ey[c + W], ex_w = Ex[c + W];           // first cell past the vector
+    V hx = ldv<T, W>(Hx + c), hy = ldv<T, W>(Hy + c), hz = ldv<T, W>(Hz + c);
+    V c0, c1, c2, c3, c4, c5, c6, c7, c8;
+    coeffs<T, CM, W>(g.ct, tc, 0, 0, c, g.C[CHXH], g.C[CHXEY], g.C[CHXEZ], c0, c1, c2);
+    coeffs<T, CM, W>(g.ct, tc, 0, 1, c, g.C[CHYH], g.C[CHYEZ], g.C[CHYEX], c3, c4, c5);
+    coeffs<T, CM, W>(g.ct, tc, 0, 2, c, g.C[CHZH], g.C[CHZEX], g.C[CHZEY], c6, c7, c8);
+#pragma unroll
+    for (int e = 0; e < W; ++e) {
+      const int k = k0 + e;
+      const T ey_k = e + 1 < W ? ey_i[e + 1 < W ? e + 1 : 0] : ey_w;
+      const T ex_k = e + 1 < W ? ex_i[e + 1 < W ? e + 1 : 0] : ex_w;
+      if (j < ny && k < nz) {                              // Hx (nx+1, ny, nz)
+        const T dEy = ey_k - ey_i[e], dEz = ez_j[e] - ez_i[e];
+        T h = c0[e] * hx[e] + c1[e] * dEy + c2[e] * dEz;
+        hx[e] = h;
+      }
+      if (inner && k < nz) {                               // Hy (nx, ny+1, nz)
+        const T dEz = ez_n[e] - ez_i[e], dEx = ex_k - ex_i[e];
+        T h = c3[e] * hy[e] + c4[e] * dEz + c5[e] * dEx;
+        hy[e] = h;
+      }
+      if (inner && j < ny && k <= nz) {                    // Hz (nx, ny, nz+1)
+        const T dEx = ex_j[e] - ex_i[e], dEy = ey_n[e] - ey_i[e];
+        T h = c6[e] * hz[e] + c7[e] * dEx + c8[e] * dEy;
+        hz[e] = h;
+      }
+    }
+    stv<T, W>(Hx + c, hx);
+    if (inner) { stv<T, W>(Hy + c, hy); stv<T, W>(Hz + c, hz); }
+    ey_i = ey_n;
+    ez_i = ez_n;
+  }
+}
+
+template <typename T, int CM, int W>
+__global__ void __launch_bounds__(256) k_update_h(Grid<T> g, MarchGeom m) {
+  __shared__ TableCache<T, CM> tc;
+  int col, ch;
+  if (!march_tile(m, col, ch)) return;
+  load_tables<T, CM>(g, 0, tc);
+  update_h_body<T, CM, W>(g.Hx, g.Hy, g.Hz, g.Ex, g.Ey, g.Ez, g, m, tc, col, ch);
+}
+
+// E bulk update; W consecutive cells of a row per thread.  With ghost != 0 (reference
+// boundary) the E planes are i in [0,nx) and the thread at i = 0 also updates Ey/Ez on
+// the i = nx wall the way the reference's updateE does: Ey there differences
+// "Hz[nx]", which is Ex(0,j,k) after this step's bulk update (the CPML terms come
+// later, in k_cpml), and Ez differences "Hy[nx]", which is the reference-layout flat
+// element j*nz+k of Hz.  With ghost == 0 the wall is left to the CPML terms.
+// Elements the bulk does not update keep their value.
+template <typename T, int CM, int W>
+__device__ __forceinline__ void update_e_body(
+    T *__restrict__ Ex, T *__restrict__ Ey, T *__restrict__ Ez,
+    const T *__restrict__ Hx, const T *__restrict__ Hy, const T *__restrict__ Hz,
+    const Grid<T> &g, const MarchGeom &m, const TableCache<T, CM> &tc, int col, int ch, int ghost) {
+#pragma clang fp contract(off)
+  typedef Vec<T, W> V;
+  const int p = (col * 256 + (int)threadIdx.x) * W;
+  const int nx = g.nx, ny = g.ny, nz = g.nz, SJ = g.SJ, SI = g.SI;
+  const int j = p / SJ;
+  const int k0 = p - j * SJ;
+  if (j > ny) return;                                      // plane padding
+  const int i0 = ch * m.chunk;
+  const int i1 = min(i0 + m.chunk, nx);
+  const bool jin = j >= 1 && j < ny;
+  int c = i0 * SI + p;
+  V hz_m = ldv<T, W>(Hz + c - SI), hy_m = ldv<T, W>(Hy + c - SI);   // H at plane i-1
+  for (int i = i0; i < i1; ++i, c += SI) {
+    const bool iin = i >= 1 && i < nx;
+    const V hz_i = ldv<T, W>(Hz + c), hy_i = ldv<T, W>(Hy + c), hx_i = ldv<T, W>(Hx + c);
+    const V hz_j = ldv<T, W>(Hz + c - SJ), hx_j = ldv<T, W>(Hx + c - SJ);
+    const T hy_w = Hy[c - 1], hx_w = Hx[c - 1];            // last cell before the vector
+    V ex = ldv<T, W>(Ex + c), ey = ldv<T, W>(Ey + c), ez = ldv<T, W>(Ez + c);
+    V c0, c1, c2, c3, c4, c5, c6, c7, c8;
+    coeffs<T, CM, W>(g.ct, tc, 3, 0, c, g.C[CEXE], g.C[CEXHZ], g.C[CEXHY], c0, c1, c2);
+    coeffs<T, CM, W>(g.ct, tc, 3, 1, c, g.C[CEYE], g.C[CEYHX], g.C[CEYHZ], c3, c4, c5);
+    coeffs<T, CM, W>(g.ct, tc, 3, 2, c, g.C[CEZE], g.C[CEZHY], g.C[CEZHX], c6, c7, c8);
+    V exb = ex;
+#pragma unroll
+    for (int e = 0; e < W; ++e) {
+      const int k = k0 + e;
+      const bool kin = k >= 1 && k < nz;
+      const T hy_k = e > 0 ? hy_i[e > 0 ? e - 1 : 0] : hy_w;
+      const T hx_k = e > 0 ? hx_i[e > 0 ? e - 1 : 0] : hx_w;
+      if (i < nx && k <= nz) {                             // Ex (nx, ny+1, nz+1)
+        const T dHz = hz_i[e] - hz_j[e], dHy = hy_i[e] - hy_k;
+        bool t = jin && kin;
+        T v = ex[e];
+        if (t) v = c0[e] * v + c1[e] * dHz + c2[e] * dHy;
+        exb[e] = v;
+        ex[e] = v;
+      }
+      if (j < ny && k <= nz) {                             // Ey (nx+1, ny, nz+1)
+        const T dHx = hx_i[e] - hx_k, dHz = hz_i[e] - hz_m[e];
+        bool t = iin && kin;
+        T v = ey[e];
+        if (t) v = c3[e] * v + c4[e] * dHx + c5[e] * dHz;
+        ey[e] = v;
+      }
+      if (k < nz) {                                        // Ez (nx+1, ny+1, nz)
+        const T dHy = hy_i[e] - hy_m[e], dHx = hx_i[e] - hx_j[e];
+        bool t = iin && jin;
+        T v = ez[e];
+        if (t) v = c6[e] * v + c7[e] * dHy + c8[e] * dHx;
+        ez[e] = v;
+      }
+    }
+    if (i < nx) stv<T, W>(Ex + c, ex);
+    stv<T, W>(Ey + c, ey);
+    stv<T, W>(Ez + c, ez);
+
+    if (ghost && i == 0) {
+      const int cg = nx * SI + p;
+#pragma unroll
+      for (int e = 0; e < W; ++e) {
+        const int k = k0 + e;
+        const bool kin = k >= 1 && k < nz;
+        if (j < ny && kin) {                               // Ey(nx, j, k)
+          const T dHx = Hx[cg + e] - Hx[cg + e - 1];
+          const T dHz = exb[e] - Hz[cg + e - SI];
+          T v = g.C[CEYE][cg + e] * Ey[cg + e] + g.C[CEYHX][cg + e] * dHx + g.C[CEYHZ][cg + e] * dHz;
+          Ey[cg + e] = v;
+        }
+        if (jin && k < nz) {                               // Ez(nx, j, k)
+          const int f = j * nz + k;                        // reference flat index into Hz
+          const int fr = f / (nz + 1);
+          const T dHy = Hz[fr * SJ + (f - fr * (nz + 1))] - Hy[cg + e - SI];
+          const T dHx = Hx[cg + e] - Hx[cg + e - SJ];
+          T v = g.C[CEZE][cg + e] * Ez[cg + e] + g.C[CEZHY][cg + e] * dHy + g.C[CEZHX][cg + e] * dHx;
+          Ez[cg + e] = v;
+        }
+      }
+    }
+    hz_m = hz_i;
+    hy_m = hy_i;
+  }
+}
+
+template <typename T, int CM, int W>
+__global__ void __launch_bounds__(256) k_update_e(Grid<T> g, MarchGeom m, int ghost) {
+  __shared__ TableCache<T, CM> tc;
+  int col, ch;
+  if (!march_tile(m, col, ch)) return;
+  load_tables<T, CM>(g, 3, tc);
+  update_e_body<T, CM, W>(g.Ex, g.Ey, g.Ez, g.Hx, g.Hy, g.Hz, g, m, tc, col, ch, ghost);
+}
+
+// ------------------------------------------------------------------ CPML pass --
+
+// The cells any CPML face of a phase touches, as disjoint boxes: the x-slabs over the
+// full y,z extent, then the y-slabs outside the x-slabs, then the z-slabs outside
+// both.  Boxes are enumerated k-fastest through a flat index.
+constexpr int MAX_CPML_BOXES = 32;
+struct CpmlBoxes {
+  int nbox;
+  long long start[MAX_CPML_BOXES + 1];   // flat start of each box; start[nbox] = total cells
+  int lo[MAX_CPML_BOXES][3], ext[MAX_CPML_BOXES][3];
+};
+
+// Add the convolution term of face axis AXIS, slot SLOT to `val` when (i,j,k) is in
+// one of that axis' faces; `diff` computes the curl difference on demand.
+template <typename T, int AXIS, int SLOT, bool MAG, typename DiffFn>
+__device__ __forceinline__ T cpml_apply(const FaceSet<T> &fs, T val, int i, int j, int k, int d1, int d2,
+                                        bool &touched, DiffFn diff) {
 #pragma unroll
   for (int s = 0; s < 2; ++s) {
     const Face<T> &F = fs.f[2 * AXIS + s];
@@ -72,121 +395,94 @@ __device__ __forceinline__ T cpml_term(const FaceSet<T> &fs, T val, T diff,
       const int q = psi_offset<AXIS>(i, j, k, ci, F.nc, d1, d2);
       const T b = MAG ? F.b_m[ci] : F.b_e[ci];
       const T a = MAG ? F.a_m[ci] : F.a_e[ci];
-      const T ps = b * F.psi[SLOT][q] + a * diff;
+      const T ps = b * F.psi[SLOT][q] + a * diff();
       F.psi[SLOT][q] = ps;
       val = val + F.cpsi[SLOT][q] * ps;
+      touched = true;
     }
   }
   return val;
 }
 
-// H update, bulk and magnetic CPML fused, one thread per (i, flat plane offset p).
-// Hx over i in [0,nx], Hy/Hz over i in [0,nx).  Per component the CPML faces are
-// applied in the reference's face order (x, then y, then z).
-template <typename T>
-__global__ void __launch_bounds__(256) k_update_h(Grid<T> g, FaceSet<T> fs) {
-#pragma clang fp contract(off)
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  const int i = blockIdx.y;
-  if (p >= g.SI) return;
-  const int j = p / g.SJ;
-  const int k = p - j * g.SJ;
-  const int nx = g.nx, ny = g.ny, nz = g.nz;
-  const int c = i * g.SI + p;
+__device__ __forceinline__ bool cpml_cell(const CpmlBoxes &bx, long long q, int &i, int &j, int &k) {
+  if (q >= bx.start[bx.nbox]) return false;
+  int b = 0;
+  while (q >= bx.start[b + 1]) ++b;
+  long long r = q - bx.start[b];
+  const int ek = bx.ext[b][2], ej = bx.ext[b][1];
+  k = bx.lo[b][2] + (int)(r % ek);
+  r /= ek;
+  j = bx.lo[b][1] + (int)(r % ej);
+  i = bx.lo[b][0] + (int)(r / ej);
+  return true;
+}
 
-  if (j < ny && k < nz) {                                  // Hx (nx+1, ny, nz)
-    const T dEy = g.Ey[c + 1] - g.Ey[c];
-    const T dEz = g.Ez[c + g.SJ] - g.Ez[c];
-    T h = g.C[CHXH][c] * g.Hx[c] + g.C[CHXEY][c] * dEy + g.C[CHXEZ][c] * dEz;
-    h = cpml_term<T, 1, 0, true>(fs, h, dEz, i, j, k, ny, nz);
-    h = cpml_term<T, 2, 0, true>(fs, h, dEy, i, j, k, ny, nz);
-    g.Hx[c] = h;
+// Magnetic CPML terms of every H element in the CPML slabs, faces in the reference's
+// order per component (x, then y, then z; cpml.js:456-724).  Runs after the H bulk
+// update, as the reference's updateMagneticCPML does; the E differences are this
+// step's E, which no kernel changes in between.
+template <typename T>
+__global__ void __launch_bounds__(256) k_cpml_h(Grid<T> g, FaceSet<T> fs, CpmlBoxes bx) {
+#pragma clang fp contract(off)
+  int i, j, k;
+  if (!cpml_cell(bx, (long long)blockIdx.x * blockDim.x + threadIdx.x, i, j, k)) return;
+  const int nx = g.nx, ny = g.ny, nz = g.nz, SJ = g.SJ, SI = g.SI;
+  const int c = i * SI + j * SJ + k;
+  const T *Ex = g.Ex, *Ey = g.Ey, *Ez = g.Ez;
+  if (j < ny && k < nz) {                                  // Hx: y then z faces
+    bool t = false;
+    T h = g.Hx[c];
+    h = cpml_apply<T, 1, 0, true>(fs, h, i, j, k, ny, nz, t, [&] { return Ez[c + SJ] - Ez[c]; });
+    h = cpml_apply<T, 2, 0, true>(fs, h, i, j, k, ny, nz, t, [&] { return Ey[c + 1] - Ey[c]; });
+    if (t) g.Hx[c] = h;
   }
-  if (i < nx) {
-    if (k < nz) {                                          // Hy (nx, ny+1, nz)
-      const T dEz = g.Ez[c + g.SI] - g.Ez[c];
-      const T dEx = g.Ex[c + 1] - g.Ex[c];
-      T h = g.C[CHYH][c] * g.Hy[c] + g.C[CHYEZ][c] * dEz + g.C[CHYEX][c] * dEx;
-      h = cpml_term<T, 0, 0, true>(fs, h, dEz, i, j, k, ny + 1, nz);
-      h = cpml_term<T, 2, 1, true>(fs, h, dEx, i, j, k, ny + 1, nz);
-      g.Hy[c] = h;
-    }
-    if (j < ny) {                                          // Hz (nx, ny, nz+1)
-      const T dEx = g.Ex[c + g.SJ] - g.Ex[c];
-      const T dEy = g.Ey[c + g.SI] - g.Ey[c];
-      T h = g.C[CHZH][c] * g.Hz[c] + g.C[CHZEX][c] * dEx + g.C[CHZEY][c] * dEy;
-      h = cpml_term<T, 0, 1, true>(fs, h, dEy, i, j, k, ny, nz + 1);
-      h = cpml_term<T, 1, 1, true>(fs, h, dEx, i, j, k, ny, nz + 1);
-      g.Hz[c] = h;
-    }
+  if (i < nx && k < nz) {                                  // Hy: x then z faces
+    bool t = false;
+    T h = g.Hy[c];
+    h = cpml_apply<T, 0, 0, true>(fs, h, i, j, k, ny + 1, nz, t, [&] { return Ez[c + SI] - Ez[c]; });
+    h = cpml_apply<T, 2, 1, true>(fs, h, i, j, k, ny + 1, nz, t, [&] { return Ex[c + 1] - Ex[c]; });
+    if (t) g.Hy[c] = h;
+  }
+  if (i < nx && j < ny) {                                  // Hz: x then y faces
+    bool t = false;
+    T h = g.Hz[c];
+    h = cpml_apply<T, 0, 1, true>(fs, h, i, j, k, ny, nz + 1, t, [&] { return Ey[c + SI] - Ey[c]; });
+    h = cpml_apply<T, 1, 1, true>(fs, h, i, j, k, ny, nz + 1, t, [&] { return Ex[c + SJ] - Ex[c]; });
+    if (t) g.Hz[c] = h;
   }
 }
 
-// E update, bulk and electric CPML fused.  With ghost != 0 (reference boundary) the
-// grid covers i in [0,nx) and the thread at i = 0 also updates Ey/Ez on the i = nx
-// wall the way the reference's updateE does: Ey there differences "Hz[nx]", which is
-// Ex(0,j,k) after this step's bulk update and before its CPML terms, and Ez
-// differences "Hy[nx]", which is the reference-layout flat element j*nz+k of Hz.
-// With ghost == 0 the grid covers i in [0,nx] and the wall gets CPML terms only.
+// Electric CPML terms (cpml.js:727-1003), after the E bulk update.  Covers the whole
+// E extents, so the i = nx wall of Ey/Ez gets its z / y terms here whichever
+// boundary mode updated (or skipped) its bulk value.
 template <typename T>
-__global__ void __launch_bounds__(256) k_update_e(Grid<T> g, FaceSet<T> fs, int ghost) {
+__global__ void __launch_bounds__(256) k_cpml_e(Grid<T> g, FaceSet<T> fs, CpmlBoxes bx) {
 #pragma clang fp contract(off)
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  const int i = blockIdx.y;
-  if (p >= g.SI) return;
-  const int j = p / g.SJ;
-  const int k = p - j * g.SJ;
-  const int nx = g.nx, ny = g.ny, nz = g.nz;
-  const int SJ = g.SJ, SI = g.SI;
-  const int c = i * SI + p;
-  const bool kin = k >= 1 && k < nz, jin = j >= 1 && j < ny, iin = i >= 1 && i < nx;
-
-  T exb = T(0);
-  if (i < nx) {                                            // Ex (nx, ny+1, nz+1)
-    const T dHz = jin ? g.Hz[c] - g.Hz[c - SJ] : T(0);
-    const T dHy = kin ? g.Hy[c] - g.Hy[c - 1] : T(0);
+  int i, j, k;
+  if (!cpml_cell(bx, (long long)blockIdx.x * blockDim.x + threadIdx.x, i, j, k)) return;
+  const int nx = g.nx, ny = g.ny, nz = g.nz, SJ = g.SJ, SI = g.SI;
+  const int c = i * SI + j * SJ + k;
+  const T *Hx = g.Hx, *Hy = g.Hy, *Hz = g.Hz;
+  if (i < nx) {                                            // Ex: y then z faces
+    bool t = false;
     T e = g.Ex[c];
-    if (jin && kin) e = g.C[CEXE][c] * e + g.C[CEXHZ][c] * dHz + g.C[CEXHY][c] * dHy;
-    exb = e;
-    e = cpml_term<T, 1, 3, false>(fs, e, dHz, i, j, k, ny + 1, nz + 1);
-    e = cpml_term<T, 2, 2, false>(fs, e, dHy, i, j, k, ny + 1, nz + 1);
-    if (j <= ny && k <= nz) g.Ex[c] = e;
+    e = cpml_apply<T, 1, 3, false>(fs, e, i, j, k, ny + 1, nz + 1, t, [&] { return Hz[c] - Hz[c - SJ]; });
+    e = cpml_apply<T, 2, 2, false>(fs, e, i, j, k, ny + 1, nz + 1, t, [&] { return Hy[c] - Hy[c - 1]; });
+    if (t) g.Ex[c] = e;
   }
-  if (j < ny) {                                            // Ey (nx+1, ny, nz+1)
-    const T dHx = kin ? g.Hx[c] - g.Hx[c - 1] : T(0);
-    const T dHz = iin ? g.Hz[c] - g.Hz[c - SI] : T(0);
+  if (j < ny) {                                            // Ey: x then z faces
+    bool t = false;
     T e = g.Ey[c];
-    if (iin && kin) e = g.C[CEYE][c] * e + g.C[CEYHX][c] * dHx + g.C[CEYHZ][c] * dHz;
-    e = cpml_term<T, 0, 2, false>(fs, e, dHz, i, j, k, ny, nz + 1);
-    e = cpml_term<T, 2, 3, false>(fs, e, dHx, i, j, k, ny, nz + 1);
-    g.Ey[c] = e;
+    e = cpml_apply<T, 0, 2, false>(fs, e, i, j, k, ny, nz + 1, t, [&] { return Hz[c] - Hz[c - SI]; });
+    e = cpml_apply<T, 2, 3, false>(fs, e, i, j, k, ny, nz + 1, t, [&] { return Hx[c] - Hx[c - 1]; });
+    if (t) g.Ey[c] = e;
   }
-  if (k < nz) {                                            // Ez (nx+1, ny+1, nz)
-    const T dHy = iin ? g.Hy[c] - g.Hy[c - SI] : T(0);
-    const T dHx = jin ? g.Hx[c] - g.Hx[c - SJ] : T(0);
+  if (k < nz) {                                            // Ez: x then y faces
+    bool t = false;
     T e = g.Ez[c];
-    if (iin && jin) e = g.C[CEZE][c] * e + g.C[CEZHY][c] * dHy + g.C[CEZHX][c] * dHx;
-    e = cpml_term<T, 0, 3, false>(fs, e, dHy, i, j, k, ny + 1, nz);
-    e = cpml_term<T, 1, 2, false>(fs, e, dHx, i, j, k, ny + 1, nz);
-    g.Ez[c] = e;
-  }
-
-  if (ghost && i == 0) {
-    const int cg = nx * SI + p;
-    if (j < ny && kin) {                                   // Ey(nx, j, k)
-      const T dHx = g.Hx[cg] - g.Hx[cg - 1];
-      const T dHz = exb - g.Hz[cg - SI];
-      T e = g.C[CEYE][cg] * g.Ey[cg] + g.C[CEYHX][cg] * dHx + g.C[CEYHZ][cg] * dHz;
-      e = cpml_term<T, 2, 3, false>(fs, e, dHx, nx, j, k, ny, nz + 1);
-      g.Ey[cg] = e;
-    }
-    if (jin && k < nz) {                                   // Ez(nx, j, k)
-      const T dHy = g.Hz[j * nz + k] - g.Hy[cg - SI];
-      const T dHx = g.Hx[cg] - g.Hx[cg - SJ];
-      T e = g.C[CEZE][cg] * g.Ez[cg] + g.C[CEZHY][cg] * dHy + g.C[CEZHX][cg] * dHx;
-      e = cpml_term<T, 1, 2, false>(fs, e, dHx, nx, j, k, ny + 1, nz);
-      g.Ez[cg] = e;
-    }
+    e = cpml_apply<T, 0, 3, false>(fs, e, i, j, k, ny + 1, nz, t, [&] { return Hy[c] - Hy[c - SI]; });
+    e = cpml_apply<T, 1, 2, false>(fs, e, i, j, k, ny + 1, nz, t, [&] { return Hx[c] - Hx[c - SJ]; });
+    if (t) g.Ez[c] = e;
   }
 }
 

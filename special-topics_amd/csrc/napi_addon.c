@@ -20,7 +20,7 @@
  *   readField(h, which, Float64Array)
  *   readFarfield(h, which, Float64Array re, Float64Array im)
  *   readPortDft(h, Float64Array re, Float64Array im)
- *   setTiming(h, bool); kernelTimes(h) -> {ms:[4], launches:[4]}
+ *   setTiming(h, bool); kernelTimes(h) -> {ms:[6], launches:[6]}
  *   destroy(h)
  */
 #define NAPI_VERSION 8
@@ -352,9 +352,9 @@ static napi_value KernelTimes(napi_env env, napi_callback_info info) {
   if (fdtd_hip_get_kernel_times(c, &t) != FDTD_HIP_OK) return throw_lib(env, "kernelTimes");
   napi_value obj, ms, la, x;
   CALL(env, napi_create_object(env, &obj));
-  CALL(env, napi_create_array_with_length(env, 4, &ms));
-  CALL(env, napi_create_array_with_length(env, 4, &la));
-  for (uint32_t q = 0; q < 4; ++q) {
+  CALL(env, napi_create_array_with_length(env, 6, &ms));
+  CALL(env, napi_create_array_with_length(env, 6, &la));
+  for (uint32_t q = 0; q < 6; ++q) {
     CALL(env, napi_create_double(env, t.ms[q], &x));
     CALL(env, napi_set_element(env, ms, q, x));
     CALL(env, napi_create_double(env, (double)t.launches[q], &x));

@@ -33,8 +33,13 @@ class FdtdHipDims(C.Structure):
                 ("numberOfTimeSteps", C.c_int), ("precision", C.c_int), ("boundary", C.c_int)]
 
 
+class FdtdHipTuning(C.Structure):
+    _fields_ = [("planes_per_workgroup", C.c_int), ("cells_per_thread", C.c_int),
+                ("coefficient_storage", C.c_int)]
+
+
 class FdtdHipKernelTimes(C.Structure):
-    _fields_ = [("ms", C.c_double * 4), ("launches", C.c_longlong * 4)]
+    _fields_ = [("ms", C.c_double * 6), ("launches", C.c_longlong * 6)]
 
 
 # Every symbol include/fdtd_hip.h declares.
@@ -44,7 +49,8 @@ EXPORTS = [
     "fdtd_hip_upload_svoltage", "fdtd_hip_upload_scurrent", "fdtd_hip_upload_farfield",
     "fdtd_hip_upload_port_dft", "fdtd_hip_run_batch", "fdtd_hip_read_field", "fdtd_hip_write_field",
     "fdtd_hip_read_psi", "fdtd_hip_write_psi", "fdtd_hip_read_farfield", "fdtd_hip_read_port_dft",
-    "fdtd_hip_set_timing", "fdtd_hip_get_kernel_times", "fdtd_hip_synchronize", "fdtd_hip_destroy",
+    "fdtd_hip_set_tuning", "fdtd_hip_get_tuning", "fdtd_hip_autotune",
+    "fdtd_hip_get_coefficient_storage", "fdtd_hip_set_timing", "fdtd_hip_get_kernel_times", "fdtd_hip_synchronize", "fdtd_hip_destroy",
 ]
 
 
@@ -93,6 +99,10 @@ def lib():
         L.fdtd_hip_write_psi.argtypes = [P, I, I, D, LL]
         L.fdtd_hip_read_farfield.argtypes = [P, I, D, D, LL]
         L.fdtd_hip_read_port_dft.argtypes = [P, D, D, LL]
+        L.fdtd_hip_set_tuning.argtypes = [P, C.POINTER(FdtdHipTuning)]
+        L.fdtd_hip_get_tuning.argtypes = [P, C.POINTER(FdtdHipTuning)]
+        L.fdtd_hip_autotune.argtypes = [P, I, C.POINTER(FdtdHipTuning)]
+        L.fdtd_hip_get_coefficient_storage.argtypes = [P, IP, IP]
         L.fdtd_hip_set_timing.argtypes = [P, I]
         L.fdtd_hip_get_kernel_times.argtypes = [P, C.POINTER(FdtdHipKernelTimes)]
         L.fdtd_hip_synchronize.argtypes = [P]

@@ -106,6 +106,7 @@ def build_problem(scenario: str, out: str, steps: int | None = None, fp32: bool 
 
 def load_scenario_problem(scenario: str, workdir: str, steps: int | None = None,
                           fp32: bool = False, no_phase: bool = False) -> Problem:
+    os.makedirs(workdir, exist_ok=True)
     tag = os.path.basename(scenario).replace(".json", "")
     out = os.path.join(workdir, f"{tag}-{steps or 'n'}-{'f32' if fp32 else 'f64'}.fdtdp")
     if not os.path.exists(out):

@@ -166,6 +166,26 @@ class HipSim:
         native.check(self.L.fdtd_hip_read_port_dft(self.h, _d(re), _d(im), n), "read_port_dft")
         return (re + 1j * im).reshape(self.n_sv + self.n_si, self.ndft)
 
+    def set_tuning(self, planes_per_workgroup: int = 0, cells_per_thread: int = 0, coefficient_storage: int = 0):
+        t = native.FdtdHipTuning(int(planes_per_workgroup), int(cells_per_thread), int(coefficient_storage))
+        native.check(self.L.fdtd_hip_set_tuning(self.h, C.byref(t)), "set_tuning")
+
+    def tuning(self):
+        t = native.FdtdHipTuning()
+        native.check(self.L.fdtd_hip_get_tuning(self.h, C.byref(t)), "get_tuning")
+        return t.planes_per_workgroup, t.cells_per_thread, t.coefficient_storage
+
+    def autotune(self, steps_per_candidate: int = 3):
+        t = native.FdtdHipTuning()
+        native.check(self.L.fdtd_hip_autotune(self.h, int(steps_per_candidate), C.byref(t)), "autotune")
+        return t.planes_per_workgroup, t.cells_per_thread, t.coefficient_storage
+
+    def coefficient_storage(self):
+        mode = C.c_int(0)
+        n = (C.c_int * 6)()
+        native.check(self.L.fdtd_hip_get_coefficient_storage(self.h, C.byref(mode), n), "get_coefficient_storage")
+        return mode.value, list(n)
+
     def set_timing(self, enable: bool):
         native.check(self.L.fdtd_hip_set_timing(self.h, 1 if enable else 0), "set_timing")
 

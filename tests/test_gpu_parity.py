@@ -41,10 +41,53 @@ def _check_run_golden(sim, fx, fields=True):
     return v, i
 
 
-def test_golden_small_fp64_matches_reference_run(get_problem):
+@pytest.mark.parametrize("storage,expect", [(0, 1), (1, 0)])
+@pytest.mark.parametrize("cells", [1, 2])
+def test_golden_small_fp64_matches_reference_run(get_problem, storage, expect, cells):
+    """Every coefficient storage (deduplicated indices, per-cell arrays) and launch
+    shape reproduces the reference run bit for bit."""
     prob = get_problem("golden-small")
     fx = G.load("run-golden-small.json")
     sim = _sim(prob)
+    sim.set_tuning(0, cells, storage)
+    mode, entries = sim.coefficient_storage()
+    assert mode == expect and (expect == 0 or 0 < max(entries) <= 256)
+    _check_run_golden(sim, fx)
+    sim.close()
+
+
+@pytest.mark.parametrize("planes,cells", [(1, 1), (3, 2), (64, 1)])
+def test_march_depth_does_not_change_results(get_problem, planes, cells):
+    prob = get_problem("golden-small")
+    fx = G.load("run-golden-small.json")
+    sim = _sim(prob)
+    sim.set_tuning(planes, cells, 0)
+    _check_run_golden(sim, fx, fields=False)
+    sim.close()
+
+
+@pytest.mark.parametrize("cells", [1, 2, 4])
+def test_fp32_launch_shapes_agree(get_problem, cells):
+    """FP32 results do not depend on the launch shape either (same arithmetic)."""
+    prob = get_problem("golden-small")
+    ref = _sim(prob, precision="fp32")
+    ref.set_tuning(1, 1, 1)
+    v0, i0 = ref.run(0, 100)
+    ref.close()
+    sim = _sim(prob, precision="fp32")
+    sim.set_tuning(5, cells, 0)
+    v1, i1 = sim.run(0, 100)
+    sim.close()
+    assert np.array_equal(v0, v1) and np.array_equal(i0, i1)
+
+
+def test_autotune_then_run_matches_reference(get_problem):
+    """The autotuner runs candidate shapes on the live problem and then resets it."""
+    prob = get_problem("golden-small")
+    fx = G.load("run-golden-small.json")
+    sim = _sim(prob)
+    planes, cells, storage = sim.autotune(2)
+    assert planes >= 1 and cells in (1, 2) and storage in (0, 1)
     _check_run_golden(sim, fx)
     sim.close()
 
