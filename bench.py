@@ -68,6 +68,18 @@ def cpu_baseline(problem64, steps: int, threads: int, target_s: float = 15.0):
     return nx * ny * nz * steps / dt / 1e6, dt, steps
 
 
+def measured_traffic(workload: str, precision: str, kernel: str):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary of this workload (calibrated against a 1 GiB copy), or None."""
+    path = os.path.join(REPO, "profiles", f"{workload}-{precision}-traffic.json")
+    try:
+        with open(path) as f:
+            k = json.load(f)["kernels"].get(kernel)
+    except (OSError, ValueError, KeyError):
+        return None
+    return int(k["bytes"]) if k else None
+
+
 def host_cores() -> int:
     try:
         return len(os.sched_getaffinity(0))
@@ -141,6 +153,7 @@ def main():
     sim.set_timing(True)
     sim.run_no_readback(0, args.timing_steps)
     ms, launches = sim.kernel_times()
+    coef_mode = sim.coefficient_storage()[0]
     sim.close()
     avg_ms = [m / max(n, 1) for m, n in zip(ms, launches)]
     nh, ne = psi_counts(problem)
@@ -154,6 +167,10 @@ def main():
     dom = 0 if avg_ms[0] >= avg_ms[1] else 1
     dom_bytes = bulk_bytes
     achieved = dom_bytes / (avg_ms[dom] * 1e-3) / 1e9
+
+    ctype = "float" if fp32 else "double"
+    kname = f"fdtd::k_update_{'h' if dom == 0 else 'e'}<{ctype}, {coef_mode}, {max(tuning[1], 1)}>"
+    traffic = measured_traffic(args.workload, args.precision, kname)
 
     mcells = ncells * args.steps / elapsed / 1e6
     line = {
@@ -174,12 +191,12 @@ def main():
                    "parallelism": "single GPU"},
         "roofline": {
             "bound": "hbm",
-            "kernel": "k_update_h" if dom == 0 else "k_update_e",
+            "kernel": kname,
             "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": traffic,
             "algorithmic_bytes_per_launch": dom_bytes,
             "avg_launch_ms": round(avg_ms[dom], 5),
         },

@@ -381,27 +381,50 @@ struct CpmlBoxes {
   int lo[MAX_CPML_BOXES][3], ext[MAX_CPML_BOXES][3];
 };
 
-// Add the convolution term of face axis AXIS, slot SLOT to `val` when (i,j,k) is in
-// one of that axis' faces; `diff` computes the curl difference on demand.
-template <typename T, int AXIS, int SLOT, bool MAG, typename DiffFn>
-__device__ __forceinline__ T cpml_apply(const FaceSet<T> &fs, T val, int i, int j, int k, int d1, int d2,
-                                        bool &touched, DiffFn diff) {
-#pragma unroll
-  for (int s = 0; s < 2; ++s) {
-    const Face<T> &F = fs.f[2 * AXIS + s];
-    const int coord = AXIS == 0 ? i : (AXIS == 1 ? j : k);
-    const int ci = coord - (MAG ? F.m_start : F.e_start);
-    if ((unsigned)ci < (unsigned)F.nc) {
-      const int q = psi_offset<AXIS>(i, j, k, ci, F.nc, d1, d2);
-      const T b = MAG ? F.b_m[ci] : F.b_e[ci];
-      const T a = MAG ? F.a_m[ci] : F.a_e[ci];
-      const T ps = b * F.psi[SLOT][q] + a * diff();
-      F.psi[SLOT][q] = ps;
-      val = val + F.cpsi[SLOT][q] * ps;
-      touched = true;
-    }
+// Slot (0..nc-1) of face axis AXIS that (i,j,k) falls in, for the n face (s = 0) or
+// the p face (s = 1); -1 when outside.
+template <typename T, int AXIS, bool MAG>
+__device__ __forceinline__ int cpml_slot(const FaceSet<T> &fs, int s, int i, int j, int k) {
+  const Face<T> &F = fs.f[2 * AXIS + s];
+  const int coord = AXIS == 0 ? i : (AXIS == 1 ? j : k);
+  const int ci = coord - (MAG ? F.m_start : F.e_start);
+  return (unsigned)ci < (unsigned)F.nc ? ci : -1;
+}
+
+// Psi recurrence and field correction of one face slot (cpml.js:470,484 order).
+template <typename T, int AXIS, int SLOT, bool MAG>
+__device__ __forceinline__ T cpml_step(const FaceSet<T> &fs, int s, int ci, T val, T diff,
+                                       int i, int j, int k, int d1, int d2) {
+  const Face<T> &F = fs.f[2 * AXIS + s];
+  const int q = psi_offset<AXIS>(i, j, k, ci, F.nc, d1, d2);
+  const T b = MAG ? F.b_m[ci] : F.b_e[ci];
+  const T a = MAG ? F.a_m[ci] : F.a_e[ci];
+  const T ps = b * F.psi[SLOT][q] + a * diff;
+  F.psi[SLOT][q] = ps;
+  return val + F.cpsi[SLOT][q] * ps;
+}
+
+// One component's CPML terms: faces of axis A1 (n then p) then of axis A2, each with
+// its curl difference computed on demand.  Loads and stores the field only if a face
+// applies.
+template <typename T, int A1, int S1, int A2, int S2, bool MAG, typename D1, typename D2>
+__device__ __forceinline__ void cpml_component(const FaceSet<T> &fs, T *__restrict__ F, int c,
+                                               int i, int j, int k, int d1, int d2, D1 diff1, D2 diff2) {
+  const int a0 = cpml_slot<T, A1, MAG>(fs, 0, i, j, k), a1 = cpml_slot<T, A1, MAG>(fs, 1, i, j, k);
+  const int b0 = cpml_slot<T, A2, MAG>(fs, 0, i, j, k), b1 = cpml_slot<T, A2, MAG>(fs, 1, i, j, k);
+  if (a0 < 0 && a1 < 0 && b0 < 0 && b1 < 0) return;
+  T v = F[c];
+  if (a0 >= 0 || a1 >= 0) {
+    const T d = diff1();
+    if (a0 >= 0) v = cpml_step<T, A1, S1, MAG>(fs, 0, a0, v, d, i, j, k, d1, d2);
+    if (a1 >= 0) v = cpml_step<T, A1, S1, MAG>(fs, 1, a1, v, d, i, j, k, d1, d2);
   }
-  return val;
+  if (b0 >= 0 || b1 >= 0) {
+    const T d = diff2();
+    if (b0 >= 0) v = cpml_step<T, A2, S2, MAG>(fs, 0, b0, v, d, i, j, k, d1, d2);
+    if (b1 >= 0) v = cpml_step<T, A2, S2, MAG>(fs, 1, b1, v, d, i, j, k, d1, d2);
+  }
+  F[c] = v;
 }
 
 __device__ __forceinline__ bool cpml_cell(const CpmlBoxes &bx, long long q, int &i, int &j, int &k) {
@@ -429,27 +452,15 @@ __global__ void __launch_bounds__(256) k_cpml_h(Grid<T> g, FaceSet<T> fs, CpmlBo
   const int nx = g.nx, ny = g.ny, nz = g.nz, SJ = g.SJ, SI = g.SI;
   const int c = i * SI + j * SJ + k;
   const T *Ex = g.Ex, *Ey = g.Ey, *Ez = g.Ez;
-  if (j < ny && k < nz) {                                  // Hx: y then z faces
-    bool t = false;
-    T h = g.Hx[c];
-    h = cpml_apply<T, 1, 0, true>(fs, h, i, j, k, ny, nz, t, [&] { return Ez[c + SJ] - Ez[c]; });
-    h = cpml_apply<T, 2, 0, true>(fs, h, i, j, k, ny, nz, t, [&] { return Ey[c + 1] - Ey[c]; });
-    if (t) g.Hx[c] = h;
-  }
-  if (i < nx && k < nz) {                                  // Hy: x then z faces
-    bool t = false;
-    T h = g.Hy[c];
-    h = cpml_apply<T, 0, 0, true>(fs, h, i, j, k, ny + 1, nz, t, [&] { return Ez[c + SI] - Ez[c]; });
-    h = cpml_apply<T, 2, 1, true>(fs, h, i, j, k, ny + 1, nz, t, [&] { return Ex[c + 1] - Ex[c]; });
-    if (t) g.Hy[c] = h;
-  }
-  if (i < nx && j < ny) {                                  // Hz: x then y faces
-    bool t = false;
-    T h = g.Hz[c];
-    h = cpml_apply<T, 0, 1, true>(fs, h, i, j, k, ny, nz + 1, t, [&] { return Ey[c + SI] - Ey[c]; });
-    h = cpml_apply<T, 1, 1, true>(fs, h, i, j, k, ny, nz + 1, t, [&] { return Ex[c + SJ] - Ex[c]; });
-    if (t) g.Hz[c] = h;
-  }
+  if (j < ny && k < nz)                                    // Hx: y then z faces
+    cpml_component<T, 1, 0, 2, 0, true>(fs, g.Hx, c, i, j, k, ny, nz,
+        [&] { return Ez[c + SJ] - Ez[c]; }, [&] { return Ey[c + 1] - Ey[c]; });
+  if (i < nx && k < nz)                                    // Hy: x then z faces
+    cpml_component<T, 0, 0, 2, 1, true>(fs, g.Hy, c, i, j, k, ny + 1, nz,
+        [&] { return Ez[c + SI] - Ez[c]; }, [&] { return Ex[c + 1] - Ex[c]; });
+  if (i < nx && j < ny)                                    // Hz: x then y faces
+    cpml_component<T, 0, 1, 1, 1, true>(fs, g.Hz, c, i, j, k, ny, nz + 1,
+        [&] { return Ey[c + SI] - Ey[c]; }, [&] { return Ex[c + SJ] - Ex[c]; });
 }
 
 // Electric CPML terms (cpml.js:727-1003), after the E bulk update.  Covers the whole
@@ -463,27 +474,15 @@ __global__ void __launch_bounds__(256) k_cpml_e(Grid<T> g, FaceSet<T> fs, CpmlBo
   const int nx = g.nx, ny = g.ny, nz = g.nz, SJ = g.SJ, SI = g.SI;
   const int c = i * SI + j * SJ + k;
   const T *Hx = g.Hx, *Hy = g.Hy, *Hz = g.Hz;
-  if (i < nx) {                                            // Ex: y then z faces
-    bool t = false;
-    T e = g.Ex[c];
-    e = cpml_apply<T, 1, 3, false>(fs, e, i, j, k, ny + 1, nz + 1, t, [&] { return Hz[c] - Hz[c - SJ]; });
-    e = cpml_apply<T, 2, 2, false>(fs, e, i, j, k, ny + 1, nz + 1, t, [&] { return Hy[c] - Hy[c - 1]; });
-    if (t) g.Ex[c] = e;
-  }
-  if (j < ny) {                                            // Ey: x then z faces
-    bool t = false;
-    T e = g.Ey[c];
-    e = cpml_apply<T, 0, 2, false>(fs, e, i, j, k, ny, nz + 1, t, [&] { return Hz[c] - Hz[c - SI]; });
-    e = cpml_apply<T, 2, 3, false>(fs, e, i, j, k, ny, nz + 1, t, [&] { return Hx[c] - Hx[c - 1]; });
-    if (t) g.Ey[c] = e;
-  }
-  if (k < nz) {                                            // Ez: x then y faces
-    bool t = false;
-    T e = g.Ez[c];
-    e = cpml_apply<T, 0, 3, false>(fs, e, i, j, k, ny + 1, nz, t, [&] { return Hy[c] - Hy[c - SI]; });
-    e = cpml_apply<T, 1, 2, false>(fs, e, i, j, k, ny + 1, nz, t, [&] { return Hx[c] - Hx[c - SJ]; });
-    if (t) g.Ez[c] = e;
-  }
+  if (i < nx)                                              // Ex: y then z faces
+    cpml_component<T, 1, 3, 2, 2, false>(fs, g.Ex, c, i, j, k, ny + 1, nz + 1,
+        [&] { return Hz[c] - Hz[c - SJ]; }, [&] { return Hy[c] - Hy[c - 1]; });
+  if (j < ny)                                              // Ey: x then z faces
+    cpml_component<T, 0, 2, 2, 3, false>(fs, g.Ey, c, i, j, k, ny, nz + 1,
+        [&] { return Hz[c] - Hz[c - SI]; }, [&] { return Hx[c] - Hx[c - 1]; });
+  if (k < nz)                                              // Ez: x then y faces
+    cpml_component<T, 0, 3, 1, 2, false>(fs, g.Ez, c, i, j, k, ny + 1, nz,
+        [&] { return Hy[c] - Hy[c - SI]; }, [&] { return Hx[c] - Hx[c - SJ]; });
 }
 
 // ---------------------------------------------------------------- port epilogue --
