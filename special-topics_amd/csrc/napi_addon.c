@@ -17,7 +17,9 @@
  *   uploadFarfield(h, nFreq, Int32Array box, Float64Array phase)
  *   uploadPortDft(h, Float64Array freqHz)
  *   runBatch(h, tsStart, count, Float64Array|null sv, Float64Array|null si)
- *   readField(h, which, Float64Array)
+ *   readField(h, which, Float64Array); writeField(h, which, Float64Array)
+ *   setTuning(h, {planesPerWorkgroup, cellsPerThread, coefficientStorage})
+ *   getTuning(h) -> tuning; autotune(h, stepsPerCandidate) -> tuning
  *   readFarfield(h, which, Float64Array re, Float64Array im)
  *   readPortDft(h, Float64Array re, Float64Array im)
  *   setTiming(h, bool); kernelTimes(h) -> {ms:[6], launches:[6]}
@@ -365,6 +367,73 @@ static napi_value KernelTimes(napi_env env, napi_callback_info info) {
   return obj;
 }
 
+static napi_value WriteField(napi_env env, napi_callback_info info) {
+  napi_value argv[3];
+  int which, ok;
+  size_t n;
+  if (!get_args(env, info, 3, argv)) return NULL;
+  FdtdHipContext *c = handle(env, argv[0]);
+  if (!c || !get_int(env, argv[1], &which)) return NULL;
+  double *p = typed(env, argv[2], napi_float64_array, &n, 0, &ok);
+  if (!ok) return NULL;
+  if (fdtd_hip_write_field(c, which, p, (long long)n) != FDTD_HIP_OK) return throw_lib(env, "writeField");
+  return undefined(env);
+}
+
+static napi_value tuning_object(napi_env env, const FdtdHipTuning *t) {
+  napi_value obj, x;
+  CALL(env, napi_create_object(env, &obj));
+  CALL(env, napi_create_int32(env, t->planes_per_workgroup, &x));
+  CALL(env, napi_set_named_property(env, obj, "planesPerWorkgroup", x));
+  CALL(env, napi_create_int32(env, t->cells_per_thread, &x));
+  CALL(env, napi_set_named_property(env, obj, "cellsPerThread", x));
+  CALL(env, napi_create_int32(env, t->coefficient_storage, &x));
+  CALL(env, napi_set_named_property(env, obj, "coefficientStorage", x));
+  return obj;
+}
+
+/* setTuning(h, {planesPerWorkgroup, cellsPerThread, coefficientStorage}); absent or
+ * zero members select the library defaults. */
+static napi_value SetTuning(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  if (!get_args(env, info, 2, argv)) return NULL;
+  FdtdHipContext *c = handle(env, argv[0]);
+  if (!c) return NULL;
+  FdtdHipTuning t;
+  memset(&t, 0, sizeof t);
+  const char *keys[3] = {"planesPerWorkgroup", "cellsPerThread", "coefficientStorage"};
+  int *dst[3] = {&t.planes_per_workgroup, &t.cells_per_thread, &t.coefficient_storage};
+  for (int q = 0; q < 3; ++q) {
+    bool has = false;
+    if (napi_has_named_property(env, argv[1], keys[q], &has) != napi_ok) { throw_napi(env, keys[q]); return NULL; }
+    if (has && !prop_int(env, argv[1], keys[q], dst[q])) return NULL;
+  }
+  if (fdtd_hip_set_tuning(c, &t) != FDTD_HIP_OK) return throw_lib(env, "setTuning");
+  return undefined(env);
+}
+
+static napi_value GetTuning(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  if (!get_args(env, info, 1, argv)) return NULL;
+  FdtdHipContext *c = handle(env, argv[0]);
+  if (!c) return NULL;
+  FdtdHipTuning t;
+  if (fdtd_hip_get_tuning(c, &t) != FDTD_HIP_OK) return throw_lib(env, "getTuning");
+  return tuning_object(env, &t);
+}
+
+/* autotune(h, stepsPerCandidate) -> chosen tuning; only before the first runBatch. */
+static napi_value Autotune(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  int steps;
+  if (!get_args(env, info, 2, argv)) return NULL;
+  FdtdHipContext *c = handle(env, argv[0]);
+  if (!c || !get_int(env, argv[1], &steps)) return NULL;
+  FdtdHipTuning t;
+  if (fdtd_hip_autotune(c, steps, &t) != FDTD_HIP_OK) return throw_lib(env, "autotune");
+  return tuning_object(env, &t);
+}
+
 static napi_value Destroy(napi_env env, napi_callback_info info) {
   napi_value argv[1];
   if (!get_args(env, info, 1, argv)) return NULL;
@@ -389,6 +458,10 @@ static napi_value Register(napi_env env, napi_value exports) {
     {"readField", NULL, ReadField, NULL, NULL, NULL, napi_enumerable, NULL},
     {"readFarfield", NULL, ReadFarfield, NULL, NULL, NULL, napi_enumerable, NULL},
     {"readPortDft", NULL, ReadPortDft, NULL, NULL, NULL, napi_enumerable, NULL},
+    {"writeField", NULL, WriteField, NULL, NULL, NULL, napi_enumerable, NULL},
+    {"setTuning", NULL, SetTuning, NULL, NULL, NULL, napi_enumerable, NULL},
+    {"getTuning", NULL, GetTuning, NULL, NULL, NULL, napi_enumerable, NULL},
+    {"autotune", NULL, Autotune, NULL, NULL, NULL, napi_enumerable, NULL},
     {"setTiming", NULL, SetTiming, NULL, NULL, NULL, napi_enumerable, NULL},
     {"kernelTimes", NULL, KernelTimes, NULL, NULL, NULL, napi_enumerable, NULL},
     {"destroy", NULL, Destroy, NULL, NULL, NULL, napi_enumerable, NULL},

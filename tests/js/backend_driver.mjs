@@ -1,0 +1,54 @@
+// Test driver: build a scenario, run it through the backend registry, and write the
+// run record (traces, final-state hashes, post-processing) for comparison with the
+// golden records.
+//
+//   node backend_driver.mjs job.json
+//
+// job = { scenario, steps, backend: 'hip' | 'auto', options: {...backend options},
+//         post: bool, farfield: bool, out: "result.json" }
+
+import fs from 'fs';
+
+import { loadScenario, buildProblemFromScenario } from '../../special-topics_amd/js/problem/scenario.mjs';
+import { FF_ORDER } from '../../special-topics_amd/js/problem/farfield.mjs';
+import { FIELD_ORDER } from '../../special-topics_amd/js/problem/grid.mjs';
+import { selectBackend } from '../../special-topics_amd/js/backends/registry.mjs';
+import { b64, sha, postRecord } from './records.mjs';
+
+async function main() {
+  const job = JSON.parse(fs.readFileSync(process.argv[2], 'utf8'));
+  const sim = buildProblemFromScenario(loadScenario(job.scenario), { steps: job.steps });
+  const { name, backend } = await selectBackend(job.backend || 'hip');
+  const options = Object.assign({ batchSize: 50, readFields: true }, job.options || {});
+  const problem = { grid: sim.grid, coeffs: sim.coeffs, cpml: sim.cpml, samplers: sim.samplers,
+                    sources: sim.sources, ff: sim.farfield, options };
+  const snaps = [];
+  const t0 = Date.now();
+  for await (const s of backend.run(problem)) snaps.push(s);
+  const loopS = (Date.now() - t0) / 1000;
+  const steps = sim.grid.numberOfTimeSteps;
+  const rec = {
+    backend: name, meta: backend.meta(), loop_s: loopS, steps,
+    snapshots: { count: snaps.length, first: snaps[0], last: snaps[snaps.length - 1] },
+    v: sim.samplers.sampledVoltages.map((o) => b64(o.sampled_value)),
+    i: sim.samplers.sampledCurrents.map((o) => b64(o.sampled_value)),
+    fields: {}, ff: {},
+  };
+  for (const c of FIELD_ORDER) rec.fields[c] = sha(sim.grid.fields[c]);
+  if (sim.farfield.nFreq > 0) {
+    for (const a of FF_ORDER) rec.ff[a] = { re: sha(sim.farfield[a].re), im: sha(sim.farfield[a].im) };
+  }
+  if (options.portDftFrequencies) {
+    rec.portDft = {
+      v: problem.portDft.voltages.map((x) => [b64(x.re), b64(x.im)]),
+      i: problem.portDft.currents.map((x) => [b64(x.re), b64(x.im)]),
+    };
+  }
+  if (job.post) Object.assign(rec, postRecord(sim, !!job.farfield));
+  fs.writeFileSync(job.out, JSON.stringify(rec));
+}
+
+main().catch((err) => {
+  console.error(err && err.stack ? err.stack : err);
+  process.exit(1);
+});
