@@ -187,6 +187,10 @@ const backend = {
     const total = grid.numberOfTimeSteps;
     const sampledVoltages = (samplers && samplers.sampledVoltages) || [];
     const sampledCurrents = (samplers && samplers.sampledCurrents) || [];
+    const probes = ((samplers && samplers.sampledEFields) || []).length + ((samplers && samplers.sampledHFields) || []).length;
+    if (probes > 0) {
+      throw new Error('hip backend: point field probes (sampledEFields / sampledHFields) are not supported');
+    }
     const tSetup = Date.now();
     const { h, header, ff } = uploadProblem(A, problem, opts);
     try {
