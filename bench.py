@@ -22,6 +22,8 @@ import sys
 import tempfile
 import time
 
+import numpy as np
+
 REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(REPO, "special-topics_amd"))
 
@@ -80,6 +82,85 @@ def measured_traffic(workload: str, precision: str, kernel: str):
     return int(k["bytes"]) if k else None
 
 
+def standard_cuts():
+    """The xy (theta = 90 deg) and xz cuts, 0..359 deg, of js_fdtd/src/index.js:77-92."""
+    ang = np.arange(360) * np.pi / 180
+    th_xz = np.where(ang > np.pi, 2 * np.pi - ang, ang)
+    ph_xz = np.where(ang > np.pi, np.pi, 0.0)
+    return (np.full(360, np.pi / 2), ang), (th_xz, ph_xz)
+
+
+def s11_from_running_dft(sim, problem):
+    """S11 of the source port from the on-GPU running port DFTs (sparameters.js:23-101)."""
+    X = sim.port_dft()
+    port = next(p for p in problem.header["ports"] if p.get("is_source_port"))
+    z = port["impedance"] if isinstance(port["impedance"], (int, float)) else port["impedance"]["re"]
+    V = X[port["sampled_voltage_index"] - 1]
+    I = X[sim.n_sv + port["sampled_current_index"] - 1]
+    a = 0.5 * (V + z * I) / np.sqrt(z)
+    b = 0.5 * (V - z * I) / np.sqrt(z)
+    return b / a
+
+
+def ifa_run(problem, precision, boundary, freqs, cuts):
+    """One IFA run to its outputs: S11 at `freqs` and directivity cuts at the NTFF
+    frequencies.  Time-to-solution is the loop plus the on-GPU port DFT readback, S11
+    and far-field evaluation; the autotuner runs before the clock starts."""
+    from fdtdhip.sim import HipSim
+    sim = HipSim(problem, precision=precision, boundary=boundary, port_dft_freqs=freqs)
+    try:
+        sim.autotune(3)
+        sim.synchronize()
+        t0 = time.perf_counter()
+        sim.run_no_readback(0, problem.steps)
+        s11 = s11_from_running_dft(sim, problem)
+        prad = sim.radiated_power()
+        d = [sim.directivity(th, ph, prad) for th, ph in cuts]
+        secs = time.perf_counter() - t0
+    finally:
+        sim.close()
+    return {"s11": s11, "directivity": d, "prad": prad, "secs": secs}
+
+
+def ifa_study(workdir, scenario="ifa-config1-7000"):
+    """FP32 vs FP64 on the IFA (S11 dB, resonance, directivity) and time-to-solution,
+    with the reference's i = nx ghost plane ('reference') and without it ('matlab')."""
+    from fdtdhip.problem import load_scenario_problem
+    p = load_scenario_problem(scenario, workdir)
+    dft = p.header["dft"] or {}
+    step = dft.get("step", 20e6)
+    freqs = (np.arange(int(np.floor(dft.get("maxFreq", 10e9) / step))) + 1) * step
+    cuts = standard_cuts()
+    nx, ny, nz = p.dims
+    out = {"workload": f"{scenario} {nx}x{ny}x{nz}, {p.steps} steps, S11 at {freqs.size} f, "
+                       f"NTFF at {len(p.header['ff'].get('frequencies', []))} f"}
+    for boundary in ("matlab", "reference"):
+        r64 = ifa_run(p, "fp64", boundary, freqs, cuts)
+        r32 = ifa_run(p, "fp32", boundary, freqs, cuts)
+        db64 = 20 * np.log10(np.abs(r64["s11"]))
+        db32 = 20 * np.log10(np.abs(r32["s11"]))
+        res64, res32 = freqs[np.argmin(db64)], freqs[np.argmin(db32)]
+        ddir, dpeak = 0.0, 0.0
+        for d64, d32 in zip(r64["directivity"], r32["directivity"]):
+            for a64, a32 in zip(d64, d32):          # theta/phi components
+                for f in range(a64.shape[0]):
+                    m = a64[f] >= 1e-3 * a64[f].max()
+                    if m.any():
+                        ddir = max(ddir, float(np.max(np.abs(10 * np.log10(a32[f][m] / a64[f][m])))))
+                    dpeak = max(dpeak, abs(10 * np.log10(a32[f].max() / a64[f].max())))
+        out[f"boundary_{boundary}"] = {
+            "fp64_s": round(r64["secs"], 4), "fp32_s": round(r32["secs"], 4),
+            "fp32_speedup": round(r64["secs"] / r32["secs"], 2),
+            "fp64_mcells": round(nx * ny * nz * p.steps / r64["secs"] / 1e6, 1),
+            "fp32_mcells": round(nx * ny * nz * p.steps / r32["secs"] / 1e6, 1),
+            "s11_max_abs_err_db": float(np.max(np.abs(db32 - db64))),
+            "s11_min_db_fp64": round(float(db64.min()), 3),
+            "resonance_ghz_fp64": float(res64 / 1e9), "resonance_err_mhz": float(abs(res32 - res64) / 1e6),
+            "directivity_max_err_db_within_30db": ddir, "directivity_peak_err_db": dpeak,
+        }
+    return out
+
+
 def host_cores() -> int:
     try:
         return len(os.sched_getaffinity(0))
@@ -97,6 +178,7 @@ def main():
     ap.add_argument("--timing-steps", type=int, default=100)
     ap.add_argument("--cpu-steps", type=int, default=0, help="0 = choose for ~20 s of CPU work")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-ifa", action="store_true", help="skip the IFA FP32-vs-FP64 time-to-solution study")
     ap.add_argument("--workdir", default=os.path.join(tempfile.gettempdir(), "fdtd-bench"))
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="first run a plain device copy of this many MiB (calibrates rocprof byte counters)")
@@ -208,6 +290,9 @@ def main():
         "tuning": {"planes_per_workgroup": tuning[0], "cells_per_thread": tuning[1],
                    "coefficient_storage": ["indexed", "arrays"][tuning[2]]},
     }
+
+    if not args.no_ifa:
+        line["ifa"] = ifa_study(args.workdir)
 
     if not args.no_cpu_baseline:
         threads = max(1, min(host_cores() - 1, 15))   # the box gives one GPU a 16-core share

@@ -151,6 +151,20 @@ int fdtd_hip_read_farfield(FdtdHipContext *ctx, int which, double *re, double *i
  * voltages first, already multiplied by dt. */
 int fdtd_hip_read_port_dft(FdtdHipContext *ctx, double *re, double *im, long long n);
 
+/* Radiated power of the Huygens box at each NTFF frequency, 0.5 * Re of the surface
+ * integral of M x conj(J), from the device-resident accumulators.  Replaces the
+ * post-loop calcRadiatedPower (js_fdtd/src/farfield.js:257-314); out[nFreq].  The sum
+ * runs in parallel: equal to the sequential JavaScript to rounding, not bit for bit. */
+int fdtd_hip_radiated_power(FdtdHipContext *ctx, double *out, int nFreq);
+
+/* Directivity over nAngles (theta[a], phi[a]) directions at every NTFF frequency, from
+ * the device-resident accumulators.  Replaces calcDirectivity
+ * (js_fdtd/src/farfield.js:328-607).  freqHz[nFreq] are the NTFF frequencies;
+ * radPower[nFreq] as from fdtd_hip_radiated_power, or NULL to compute it here.
+ * dataTheta/dataPhi receive [nFreq][nAngles]. */
+int fdtd_hip_directivity(FdtdHipContext *ctx, const double *freqHz, const double *radPower, int nAngles,
+                         const double *theta, const double *phi, double *dataTheta, double *dataPhi);
+
 /* Launch shape and coefficient storage of the stencil kernels.  Every setting gives
  * bit-identical results; only speed and memory differ.  Zero fields select the
  * defaults. */

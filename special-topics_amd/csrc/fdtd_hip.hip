@@ -5,6 +5,7 @@
 
 #include "../../include/fdtd_hip.h"
 #include "fdtd_kernels.hpp"
+#include "ntff_post.hpp"
 
 #include <hip/hip_runtime.h>
 
@@ -804,6 +805,83 @@ int fdtd_hip_read_port_dft(FdtdHipContext *c, double *re, double *im, long long 
   HIP_TRY(hipMemcpyAsync(v.data(), c->dft_acc, 2 * n * 8, hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
   for (long long q = 0; q < n; ++q) { re[q] = v[2 * q] * c->d.dt; im[q] = v[2 * q + 1] * c->d.dt; }
+  return FDTD_HIP_OK;
+}
+
+extern "C++" {
+namespace {
+
+NtffPost ntff_post_params(const FdtdHipContext *c) {
+  const FarfieldState &f = c->ff;
+  NtffPost p;
+  p.acc = f.acc;
+  p.nfreq = f.nfreq; p.ni = f.ni; p.nj = f.nj; p.nk = f.nk;
+  p.li = f.LI + 1; p.lj = f.LJ + 1; p.lk = f.LK + 1;
+  p.ui = f.UI + 1; p.uj = f.UJ + 1; p.uk = f.UK + 1;
+  p.dx = c->d.dx; p.dy = c->d.dy; p.dz = c->d.dz;
+  return p;
+}
+
+// Device scratch of one post-processing call, freed when the call returns.
+struct Scratch {
+  std::vector<void *> ptrs;
+  ~Scratch() { for (void *p : ptrs) (void)hipFree(p); }
+  template <typename T>
+  T *put(FdtdHipContext *c, const T *host, size_t n) {
+    void *p = nullptr;
+    if (hipMalloc(&p, n * sizeof(T) + 16) != hipSuccess) return nullptr;
+    ptrs.push_back(p);
+    if (host && hipMemcpyAsync(p, host, n * sizeof(T), hipMemcpyHostToDevice, c->stream) != hipSuccess) return nullptr;
+    return (T *)p;
+  }
+};
+
+}  // namespace
+}  // extern "C++"
+
+int fdtd_hip_radiated_power(FdtdHipContext *c, double *out, int nFreq) {
+  if (!c || !out) return fail("radiated_power: null argument");
+  if (!c->nff) return fail("radiated_power: no far-field box uploaded");
+  if (nFreq != c->nff) return fail("radiated_power: %d frequencies, the box has %d", nFreq, c->nff);
+  Scratch s;
+  double *d = s.put<double>(c, nullptr, nFreq);
+  if (!d) return fail("radiated_power: device allocation failed");
+  hipLaunchKernelGGL(k_ntff_power, dim3(nFreq), dim3(256), 0, c->stream, ntff_post_params(c), d);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpyAsync(out, d, nFreq * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return FDTD_HIP_OK;
+}
+
+int fdtd_hip_directivity(FdtdHipContext *c, const double *freqHz, const double *radPower, int nAngles,
+                         const double *theta, const double *phi, double *dataTheta, double *dataPhi) {
+  if (!c || !freqHz || !theta || !phi || !dataTheta || !dataPhi) return fail("directivity: null argument");
+  if (!c->nff) return fail("directivity: no far-field box uploaded");
+  if (nAngles <= 0) return fail("directivity: %d angles", nAngles);
+  const int nf = c->nff;
+  std::vector<double> prad(nf), kw(nf), ang(4 * (size_t)nAngles);
+  if (radPower) {
+    std::copy(radPower, radPower + nf, prad.begin());
+  } else if (fdtd_hip_radiated_power(c, prad.data(), nf)) {
+    return FDTD_HIP_ERR;
+  }
+  for (int f = 0; f < nf; ++f) kw[f] = 2 * M_PI * freqHz[f] / 2.99792458e8;
+  for (int a = 0; a < nAngles; ++a) {
+    ang[4 * a] = std::sin(theta[a]); ang[4 * a + 1] = std::cos(theta[a]);
+    ang[4 * a + 2] = std::sin(phi[a]); ang[4 * a + 3] = std::cos(phi[a]);
+  }
+  Scratch s;
+  const double *dk = s.put(c, kw.data(), nf), *dp = s.put(c, prad.data(), nf);
+  const double *da = s.put(c, ang.data(), ang.size());
+  double *dt = s.put<double>(c, nullptr, (size_t)nf * nAngles), *dq = s.put<double>(c, nullptr, (size_t)nf * nAngles);
+  if (!dk || !dp || !da || !dt || !dq) return fail("directivity: device allocation failed");
+  constexpr int NA = 4;
+  hipLaunchKernelGGL(k_ntff_directivity<NA>, dim3((nAngles + NA - 1) / NA, nf), dim3(256), 0, c->stream,
+                     ntff_post_params(c), dk, dp, da, nAngles, dt, dq);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpyAsync(dataTheta, dt, (size_t)nf * nAngles * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipMemcpyAsync(dataPhi, dq, (size_t)nf * nAngles * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
   return FDTD_HIP_OK;
 }
 

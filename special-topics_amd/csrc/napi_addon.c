@@ -22,6 +22,8 @@
  *   getTuning(h) -> tuning; autotune(h, stepsPerCandidate) -> tuning
  *   readFarfield(h, which, Float64Array re, Float64Array im)
  *   readPortDft(h, Float64Array re, Float64Array im)
+ *   radiatedPower(h, Float64Array out[nFreq])
+ *   directivity(h, freqHz, radPower|null, theta, phi, dataTheta, dataPhi)
  *   setTiming(h, bool); kernelTimes(h) -> {ms:[6], launches:[6]}
  *   destroy(h)
  */
@@ -334,6 +336,46 @@ static napi_value ReadPortDft(napi_env env, napi_callback_info info) {
   return undefined(env);
 }
 
+static napi_value RadiatedPower(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  int ok;
+  size_t n;
+  if (!get_args(env, info, 2, argv)) return NULL;
+  FdtdHipContext *c = handle(env, argv[0]);
+  if (!c) return NULL;
+  double *out = typed(env, argv[1], napi_float64_array, &n, 0, &ok);
+  if (!ok) return NULL;
+  if (fdtd_hip_radiated_power(c, out, (int)n) != FDTD_HIP_OK) return throw_lib(env, "radiatedPower");
+  return undefined(env);
+}
+
+static napi_value Directivity(napi_env env, napi_callback_info info) {
+  napi_value argv[7];
+  int ok;
+  size_t nf, np, na, nb, nt, nq;
+  if (!get_args(env, info, 7, argv)) return NULL;
+  FdtdHipContext *c = handle(env, argv[0]);
+  if (!c) return NULL;
+  double *freq = typed(env, argv[1], napi_float64_array, &nf, 0, &ok);
+  if (!ok) return NULL;
+  double *prad = typed(env, argv[2], napi_float64_array, &np, 1, &ok);
+  if (!ok) return NULL;
+  double *th = typed(env, argv[3], napi_float64_array, &na, 0, &ok);
+  if (!ok) return NULL;
+  double *ph = typed(env, argv[4], napi_float64_array, &nb, 0, &ok);
+  if (!ok) return NULL;
+  double *dt = typed(env, argv[5], napi_float64_array, &nt, 0, &ok);
+  if (!ok) return NULL;
+  double *dp = typed(env, argv[6], napi_float64_array, &nq, 0, &ok);
+  if (!ok) return NULL;
+  if (na != nb || nt != nq || nt != nf * na || (prad && np != nf)) {
+    napi_throw_range_error(env, NULL, "directivity: array lengths do not match nFreq x nAngles");
+    return NULL;
+  }
+  if (fdtd_hip_directivity(c, freq, prad, (int)na, th, ph, dt, dp) != FDTD_HIP_OK) return throw_lib(env, "directivity");
+  return undefined(env);
+}
+
 static napi_value SetTiming(napi_env env, napi_callback_info info) {
   napi_value argv[2];
   bool on = false;
@@ -462,6 +504,8 @@ static napi_value Register(napi_env env, napi_value exports) {
     {"setTuning", NULL, SetTuning, NULL, NULL, NULL, napi_enumerable, NULL},
     {"getTuning", NULL, GetTuning, NULL, NULL, NULL, napi_enumerable, NULL},
     {"autotune", NULL, Autotune, NULL, NULL, NULL, napi_enumerable, NULL},
+    {"radiatedPower", NULL, RadiatedPower, NULL, NULL, NULL, napi_enumerable, NULL},
+    {"directivity", NULL, Directivity, NULL, NULL, NULL, napi_enumerable, NULL},
     {"setTiming", NULL, SetTiming, NULL, NULL, NULL, napi_enumerable, NULL},
     {"kernelTimes", NULL, KernelTimes, NULL, NULL, NULL, napi_enumerable, NULL},
     {"destroy", NULL, Destroy, NULL, NULL, NULL, napi_enumerable, NULL},

@@ -160,6 +160,23 @@ class HipSim:
         native.check(self.L.fdtd_hip_read_farfield(self.h, which, _d(re), _d(im), n), "read_farfield")
         return re, im
 
+    def radiated_power(self) -> np.ndarray:
+        out = np.zeros(self.nfreq)
+        native.check(self.L.fdtd_hip_radiated_power(self.h, _d(out), self.nfreq), "radiated_power")
+        return out
+
+    def directivity(self, theta, phi, rad_power=None):
+        """(dataTheta, dataPhi), each [nFreq][nAngles], at the NTFF frequencies."""
+        th = np.ascontiguousarray(theta, dtype=np.float64)
+        ph = np.ascontiguousarray(phi, dtype=np.float64)
+        fr = np.ascontiguousarray(self.problem.header["ff"]["frequencies"], dtype=np.float64)
+        dt = np.zeros((self.nfreq, th.size))
+        dp = np.zeros((self.nfreq, th.size))
+        rp = None if rad_power is None else _d(np.ascontiguousarray(rad_power, dtype=np.float64))
+        native.check(self.L.fdtd_hip_directivity(self.h, _d(fr), rp, th.size, _d(th), _d(ph), _d(dt), _d(dp)),
+                     "directivity")
+        return dt, dp
+
     def port_dft(self):
         n = (self.n_sv + self.n_si) * self.ndft
         re, im = np.zeros(n), np.zeros(n)

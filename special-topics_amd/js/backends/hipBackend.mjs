@@ -86,6 +86,7 @@ export function resolveOptions(options) {
     batchSize, precision, boundary, tuning,
     portDftFrequencies: o.portDftFrequencies ? asFloat64(o.portDftFrequencies) : null,
     readFields: !!o.readFields,
+    farfieldCuts: o.farfieldCuts || null,
     autotuneSteps: pick(o.autotuneSteps, 3),
   };
 }
@@ -155,6 +156,28 @@ function snapshot(ts, total, startTime, sampledVoltages, dt) {
   };
 }
 
+/**
+ * Radiated power and directivity cuts from the accumulators still on the device
+ * (calcRadiatedPower / calcDirectivity of farfield.js, evaluated by the GPU; equal to
+ * the sequential FP64 JavaScript to rounding).  cuts: [{theta, phi}].
+ */
+function farfieldOnDevice(A, h, ffHeader, cuts) {
+  const nf = ffHeader.nFreq;
+  const freqs = Float64Array.from(ffHeader.frequencies);
+  const radPower = new Float64Array(nf);
+  A.radiatedPower(h, radPower);
+  return {
+    radPower,
+    cuts: cuts.map((cut) => {
+      const th = asFloat64(cut.theta), ph = asFloat64(cut.phi), na = th.length;
+      const t = new Float64Array(nf * na), p = new Float64Array(nf * na);
+      A.directivity(h, freqs, radPower, th, ph, t, p);
+      const rows = (x) => Array.from({ length: nf }, (_, f) => x.subarray(f * na, (f + 1) * na));
+      return { dataTheta: rows(t), dataPhi: rows(p) };
+    }),
+  };
+}
+
 let lastRun = null;
 
 /** @type {import('./types.js').Backend} */
@@ -214,6 +237,7 @@ const backend = {
       }
       if (header.ff.nFreq > 0) {
         FF_ORDER.forEach((name, q) => A.readFarfield(h, q, ff[name].re, ff[name].im));
+        if (opts.farfieldCuts) problem.farfieldPost = farfieldOnDevice(A, h, header.ff, opts.farfieldCuts);
       }
       if (opts.portDftFrequencies && opts.portDftFrequencies.length > 0) {
         const m = opts.portDftFrequencies.length;

@@ -3,11 +3,15 @@
 //
 //   node cli.mjs [--scenario=NAME|PATH] [--backend=auto|hip] [--out=FILE]
 //                [--precision=fp64|fp32] [--boundary=reference|matlab] [--steps=N]
+//                [--farfield-post=gpu|js]
 //
 // FDTD_SCENARIO / FDTD_BACKEND / FDTD_STEPS are honoured like the reference CLI.  The
 // time loop runs on the selected backend; the port DFTs, S-parameters and NTFF
 // directivity cuts are computed afterwards in FP64 with the reference's formulas, and
-// the output has the same schema (meta, sparams, farfield, timeDomain).
+// the output has the same schema (meta, sparams, farfield, timeDomain).  With
+// --farfield-post=gpu (the default on the hip backend) the radiated power and the cuts
+// are evaluated on the GPU from the device-resident accumulators, equal to the
+// JavaScript to rounding; --farfield-post=js reproduces the reference bit for bit.
 
 import fs from 'fs';
 import path from 'path';
@@ -28,7 +32,7 @@ function arg(name, env, dflt) {
 const toDb = (x) => 10 * Math.log10(Math.max(x, 1e-10));
 
 /** Post-processing and the out.json record, as index.js:63-130 builds it. */
-export function buildResults(sim, scenarioName) {
+export function buildResults(sim, scenarioName, farfieldPost) {
   const { grid, samplers, farfield, ports, frequencies, numberOfTimeSteps, dt, dft } = sim;
   const freqArr = frequencyGrid(dft);
   computeObserverDFT(samplers.sampledVoltages, dt, 0, freqArr);
@@ -37,10 +41,15 @@ export function buildResults(sim, scenarioName) {
                            sampledCurrents: samplers.sampledCurrents, dt, freqArr });
   const ffData = [];
   if (frequencies.length > 0) {
-    const prad = calcRadiatedPower(farfield, grid);
     const cuts = standardCuts();
-    const xy = calcDirectivity(farfield, grid, prad, cuts.xy.theta, cuts.xy.phi);
-    const xz = calcDirectivity(farfield, grid, prad, cuts.xz.theta, cuts.xz.phi);
+    let xy, xz;
+    if (farfieldPost) {
+      [xy, xz] = farfieldPost.cuts;
+    } else {
+      const prad = calcRadiatedPower(farfield, grid);
+      xy = calcDirectivity(farfield, grid, prad, cuts.xy.theta, cuts.xy.phi);
+      xz = calcDirectivity(farfield, grid, prad, cuts.xz.theta, cuts.xz.phi);
+    }
     const deg = Array.from(cuts.angles, (x) => x * 180 / Math.PI);
     for (let f = 0; f < frequencies.length; f++) {
       for (const [plane, d] of [['xy', xy], ['xz', xz]]) {
@@ -87,11 +96,15 @@ async function main() {
   const { name, backend } = await selectBackend(backendArg);
   console.log(`Running FDTD on backend: ${name}`);
 
+  const ffPost = arg('farfield-post', null, name === 'hip' ? 'gpu' : 'js');
+  if (ffPost !== 'gpu' && ffPost !== 'js') throw new Error(`--farfield-post must be gpu or js, not ${ffPost}`);
+  const cuts = standardCuts();
   const problem = {
     grid: sim.grid, coeffs: sim.coeffs, cpml: sim.cpml, samplers: sim.samplers, sources: sim.sources,
     ff: sim.farfield,
     options: { batchSize: 50, precision: arg('precision', 'FDTD_HIP_PRECISION', 'fp64'),
-               boundary: arg('boundary', 'FDTD_HIP_BOUNDARY', 'reference') },
+               boundary: arg('boundary', 'FDTD_HIP_BOUNDARY', 'reference'),
+               farfieldCuts: ffPost === 'gpu' ? [cuts.xy, cuts.xz] : null },
   };
   const start = Date.now();
   for await (const snap of backend.run(problem)) {
@@ -99,7 +112,7 @@ async function main() {
   }
   const loopS = (Date.now() - start) / 1000;
   console.log(`\nSimulation complete in ${loopS}s. Processing records...`);
-  const results = buildResults(sim, scenarioName);
+  const results = buildResults(sim, scenarioName, problem.farfieldPost);
   results.meta.backend = Object.assign({ loop_s: loopS }, backend.meta ? backend.meta() : { name });
   fs.mkdirSync(path.dirname(out), { recursive: true });
   fs.writeFileSync(out, JSON.stringify(results));

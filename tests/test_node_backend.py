@@ -36,9 +36,10 @@ def node(args, env=None, timeout=600):
     return subprocess.run(["node", *args], capture_output=True, text=True, env=e, timeout=timeout)
 
 
-def run_backend(tmp_path, scenario, steps, post=True, farfield=True, options=None, env=None):
+def run_backend(tmp_path, scenario, steps, post=True, farfield=True, options=None, env=None, gpu_farfield=False):
     job = {"scenario": SCEN[scenario], "steps": steps, "backend": "hip", "post": post,
-           "farfield": farfield, "options": options or {}, "out": str(tmp_path / "rec.json")}
+           "farfield": farfield, "gpuFarfield": gpu_farfield, "options": options or {},
+           "out": str(tmp_path / "rec.json")}
     jp = tmp_path / "job.json"
     jp.write_text(json.dumps(job))
     print(f"[node backend] {scenario} steps={steps} options={options}", flush=True)
@@ -164,10 +165,29 @@ def test_hip_backend_running_port_dft(tmp_path):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("scenario", ["golden-small", "ifa-config1-7000"])
+def test_gpu_farfield_post_matches_reference(tmp_path, scenario):
+    """Radiated power and directivity cuts evaluated on the GPU from the device-resident
+    accumulators vs the reference's calcRadiatedPower/calcDirectivity outputs: parallel
+    sums and device sin/cos, so equal to 1e-10 relative (of each cut's peak), not bitwise."""
+    fx = G.load(f"run-{scenario}.json")
+    rec = run_backend(tmp_path, scenario, fx["steps"], post=False, farfield=False, gpu_farfield=True,
+                      options={"batchSize": 1000})
+    g, r = rec["gpuFarfield"], fx["farfield"]
+    prad, prad_ref = G.f64(g["prad"]), G.f64(r["prad"])
+    assert np.max(np.abs(prad - prad_ref) / np.abs(prad_ref)) <= 1e-10
+    for cut in ("xy", "xz"):
+        assert len(g[cut]) == len(r[cut])
+        for (gt, gp), (rt, rp) in zip(g[cut], r[cut]):
+            for mine, ref in ((G.f64(gt), G.f64(rt)), (G.f64(gp), G.f64(rp))):
+                assert np.max(np.abs(mine - ref)) <= 1e-10 * np.max(np.abs(ref)), cut
+
+
+@pytest.mark.gpu
 def test_cli_writes_reference_schema(tmp_path):
     out = tmp_path / "out.json"
     res = node([os.path.join(JS, "cli.mjs"), f"--scenario={SCEN['golden-small']}", "--backend=hip",
-                f"--out={out}", "--steps=160"])
+                f"--out={out}", "--steps=160", "--farfield-post=js"])
     assert res.returncode == 0, res.stderr[-3000:]
     r = json.loads(out.read_text())
     assert set(r) == {"meta", "sparams", "farfield", "timeDomain"}
