@@ -161,6 +161,10 @@ def ifa_study(workdir, scenario="ifa-config1-7000"):
     return out
 
 
+def progress(msg: str):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def host_cores() -> int:
     try:
         return len(os.sched_getaffinity(0))
@@ -179,6 +183,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=0, help="0 = choose for ~20 s of CPU work")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-ifa", action="store_true", help="skip the IFA FP32-vs-FP64 time-to-solution study")
+    ap.add_argument("--ntff-mode", default="auto", choices=["auto", "direct", "blocked"],
+                    help="NTFF running DFT: step-by-step (direct) or K-blocked on the matrix cores")
     ap.add_argument("--workdir", default=os.path.join(tempfile.gettempdir(), "fdtd-bench"))
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="first run a plain device copy of this many MiB (calibrates rocprof byte counters)")
@@ -208,11 +214,15 @@ def main():
     fp32 = args.precision == "fp32"
     total_steps = args.warmup + args.steps
     t_setup = time.perf_counter()
+    progress(f"building {args.workload}")
     problem = load_scenario_problem(args.workload, args.workdir, steps=max(total_steps, args.timing_steps + 1),
                                     fp32=fp32, no_phase=False)
     nx, ny, nz = problem.dims
     ncells = nx * ny * nz
+    progress(f"uploading {nx}x{ny}x{nz}")
     sim = HipSim(problem, precision=args.precision)
+    sim.set_ntff_mode(args.ntff_mode)
+    progress("tuning")
     if args.planes < 0:
         tuning = sim.autotune(3)
     else:
@@ -221,6 +231,7 @@ def main():
     setup_s = time.perf_counter() - t_setup
 
     # Throughput: W untimed steps, then exactly K steps between device-wide syncs.
+    progress(f"timing {args.steps} steps with {tuning}")
     sim.run_no_readback(0, args.warmup)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -231,11 +242,13 @@ def main():
 
     # Per-kernel durations with HIP events on the library stream (fresh state).
     sim = HipSim(problem, precision=args.precision)
+    sim.set_ntff_mode(args.ntff_mode)
     sim.set_tuning(*tuning)
     sim.set_timing(True)
     sim.run_no_readback(0, args.timing_steps)
     ms, launches = sim.kernel_times()
     coef_mode = sim.coefficient_storage()[0]
+    ntff_blocked = sim.ntff_blocked() if sim.nfreq else None
     sim.close()
     avg_ms = [m / max(n, 1) for m, n in zip(ms, launches)]
     nh, ne = psi_counts(problem)
@@ -285,6 +298,7 @@ def main():
         "kernels_avg_ms": {"update_h": round(avg_ms[0], 5), "cpml_h": round(avg_ms[4], 5),
                            "update_e": round(avg_ms[1], 5), "cpml_e": round(avg_ms[5], 5),
                            "port_epilogue": round(avg_ms[2], 5), "ntff": round(avg_ms[3], 5)},
+        "ntff_mode": None if ntff_blocked is None else ("blocked (matrix cores)" if ntff_blocked else "direct"),
         "hbm_equivalent_frac_step": round((bytes_h + bytes_e) / (elapsed / args.steps) / 1e9 / HBM_PEAK_GBS, 4),
         "setup_s": round(setup_s, 2),
         "tuning": {"planes_per_workgroup": tuning[0], "cells_per_thread": tuning[1],
@@ -292,9 +306,11 @@ def main():
     }
 
     if not args.no_ifa:
+        progress("IFA FP32-vs-FP64 study")
         line["ifa"] = ifa_study(args.workdir)
 
     if not args.no_cpu_baseline:
+        progress("CPU baseline")
         threads = max(1, min(host_cores() - 1, 15))   # the box gives one GPU a 16-core share
         p64 = load_scenario_problem(args.workload, args.workdir, steps=max(total_steps, args.timing_steps + 1),
                                     fp32=False, no_phase=False)

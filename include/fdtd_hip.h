@@ -165,6 +165,19 @@ int fdtd_hip_radiated_power(FdtdHipContext *ctx, double *out, int nFreq);
 int fdtd_hip_directivity(FdtdHipContext *ctx, const double *freqHz, const double *radPower, int nAngles,
                          const double *theta, const double *phi, double *dataTheta, double *dataPhi);
 
+/* How the NTFF running DFT is accumulated.  DIRECT: every accumulator of every
+ * frequency is updated on every step in the reference's exact arithmetic
+ * (farfield.js:90-248; bit-identical in FP64).  BLOCKED: each step stores its surface
+ * samples and every 64 steps a matrix-core product folds them into the accumulators
+ * (equal to DIRECT to FP64 rounding).  AUTO (default): BLOCKED for FP32 contexts,
+ * DIRECT for FP64.  No counterpart in the reference ABI. */
+#define FDTD_HIP_NTFF_AUTO 0
+#define FDTD_HIP_NTFF_DIRECT 1
+#define FDTD_HIP_NTFF_BLOCKED 2
+int fdtd_hip_set_ntff_mode(FdtdHipContext *ctx, int mode);
+/* *blocked = 1 when the context accumulates the NTFF DFT in blocked form. */
+int fdtd_hip_get_ntff_mode(FdtdHipContext *ctx, int *blocked);
+
 /* Launch shape and coefficient storage of the stencil kernels.  Every setting gives
  * bit-identical results; only speed and memory differ.  Zero fields select the
  * defaults. */

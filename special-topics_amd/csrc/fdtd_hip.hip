@@ -6,6 +6,7 @@
 #include "../../include/fdtd_hip.h"
 #include "fdtd_kernels.hpp"
 #include "ntff_post.hpp"
+#include "ntff_block.hpp"
 
 #include <hip/hip_runtime.h>
 
@@ -87,6 +88,15 @@ struct FdtdHipContext {
 
   int ndft = 0;
   double *dft_w = nullptr, *dft_acc = nullptr;
+
+  // K-blocked NTFF (matrix-core) state: sample buffer and the current block.
+  int ntff_mode = FDTD_HIP_NTFF_AUTO;
+  void *ntff_buf = nullptr;
+  long ntff_spad = 0;
+  long ntff_base[25] = {}, ntff_size[24] = {};
+  int blk_t0 = 0;          // step of row 0 of the current block
+  int blk_rows = 0;        // rows gathered into the current block
+  int blk_flushed = 0;     // rows already folded into the accumulators
 
   FdtdHipTuning tune{0, 0, 0};  // requested launch shape / coefficient storage (0 = default)
   int coef_mode = -1;           // CoefMode in use; -1 = not decided yet
@@ -414,6 +424,59 @@ CpmlBoxes cpml_boxes(const FdtdHipContext *c, int phase) {
   return bx;
 }
 
+bool ntff_blocked(const FdtdHipContext *c) {
+  if (c->nff == 0) return false;
+  return c->ntff_mode == FDTD_HIP_NTFF_BLOCKED || (c->ntff_mode == FDTD_HIP_NTFF_AUTO && c->fp32());
+}
+
+// Sample ranges of the 24 accumulators in the blocked buffer, each padded to a
+// multiple of 16 so a matrix-core tile never spans two accumulators.
+int ensure_ntff_buffer(FdtdHipContext *c) {
+  if (c->ntff_buf) return FDTD_HIP_OK;
+  const FarfieldState &f = c->ff;
+  const long sz[3] = {(long)f.nj * f.nk, (long)f.ni * f.nk, (long)f.ni * f.nj};
+  long pos = 0;
+  for (int a = 0; a < 24; ++a) {
+    c->ntff_base[a] = pos;
+    c->ntff_size[a] = sz[a / 8];
+    pos += (sz[a / 8] + 15) / 16 * 16;
+  }
+  c->ntff_base[24] = pos;
+  c->ntff_spad = pos;
+  c->blk_t0 = c->blk_rows = c->blk_flushed = 0;
+  return dev_alloc(c, &c->ntff_buf, (size_t)NTFF_K * pos * c->esz);
+}
+
+template <typename T>
+NtffBlock<T> ntff_block(const FdtdHipContext *c) {
+  NtffBlock<T> nb;
+  nb.buf = (T *)c->ntff_buf;
+  nb.spad = c->ntff_spad;
+  for (int a = 0; a < 25; ++a) nb.base[a] = c->ntff_base[a];
+  for (int a = 0; a < 24; ++a) nb.size[a] = c->ntff_size[a];
+  return nb;
+}
+
+// Fold rows [blk_flushed, blk_rows) of the current block into the accumulators.
+template <typename T>
+int ntff_flush(FdtdHipContext *c) {
+  if (c->blk_rows <= c->blk_flushed) return FDTD_HIP_OK;
+  const NtffBlock<T> nb = ntff_block<T>(c);
+  // Persistent grid: a few workgroups per CU, each looping over sample tiles.
+  const unsigned grid = (unsigned)std::min<long>((c->ntff_spad / 16 + 3) / 4, 2048);
+  for (int f0 = 0; f0 < c->nff; f0 += NTFF_FREQ_GROUP) {
+    if (2 * (c->nff - f0) <= 16)
+      hipLaunchKernelGGL((k_ntff_mfma<T, 1>), dim3(grid), dim3(256), 0, c->stream, c->ff, nb, c->blk_t0,
+                         c->blk_flushed, c->blk_rows, f0);
+    else
+      hipLaunchKernelGGL((k_ntff_mfma<T, 2>), dim3(grid), dim3(256), 0, c->stream, c->ff, nb, c->blk_t0,
+                         c->blk_flushed, c->blk_rows, f0);
+  }
+  c->blk_flushed = c->blk_rows;
+  if (c->blk_rows == NTFF_K) c->blk_t0 += NTFF_K, c->blk_rows = c->blk_flushed = 0;
+  return FDTD_HIP_OK;
+}
+
 template <typename T>
 int launch_step(FdtdHipContext *c, int ts) {
   const Grid<T> g = make_grid<T>(c);
@@ -444,7 +507,20 @@ int launch_step(FdtdHipContext *c, int ts) {
   if (t) HIP_TRY(hipEventRecord(c->ev[5], c->stream));
   if (c->nff > 0) {
     const long total = 2 * ((long)c->ff.nj * c->ff.nk + (long)c->ff.ni * c->ff.nk + (long)c->ff.ni * c->ff.nj);
-    hipLaunchKernelGGL(k_ntff_direct<T>, dim3((unsigned)((total + 255) / 256)), block, 0, c->stream, g, c->ff, ts);
+    if (ntff_blocked(c)) {
+      // Steps must follow on from the block; a jump (a new run) starts a new block.
+      if (ts != c->blk_t0 + c->blk_rows) {
+        if (ntff_flush<T>(c)) return FDTD_HIP_ERR;
+        c->blk_t0 = ts;
+        c->blk_rows = c->blk_flushed = 0;
+      }
+      hipLaunchKernelGGL(k_ntff_gather<T>, dim3((unsigned)((total + 255) / 256)), block, 0, c->stream, g, c->ff,
+                         ntff_block<T>(c), c->blk_rows);
+      ++c->blk_rows;
+      if (c->blk_rows == NTFF_K && ntff_flush<T>(c)) return FDTD_HIP_ERR;
+    } else {
+      hipLaunchKernelGGL(k_ntff_direct<T>, dim3((unsigned)((total + 255) / 256)), block, 0, c->stream, g, c->ff, ts);
+    }
   }
   if (t) HIP_TRY(hipEventRecord(c->ev[6], c->stream));
   HIP_TRY(hipGetLastError());
@@ -706,10 +782,14 @@ int fdtd_hip_run_batch(FdtdHipContext *c, int tsStart, int count, double *svOut,
   if (ensure_traces(c) || finalize_coefficients(c)) return FDTD_HIP_ERR;
   c->cpml_h = cpml_boxes(c, 0);
   c->cpml_e = cpml_boxes(c, 1);
+  if (ntff_blocked(c) && ensure_ntff_buffer(c)) return FDTD_HIP_ERR;
   for (int s = 0; s < count; ++s) {
     const int rc = c->fp32() ? launch_step<float>(c, tsStart + s) : launch_step<double>(c, tsStart + s);
     if (rc) return rc;
   }
+  // Leave the accumulators complete at the end of every batch.
+  if (ntff_blocked(c) && (c->fp32() ? ntff_flush<float>(c) : ntff_flush<double>(c))) return FDTD_HIP_ERR;
+  HIP_TRY(hipGetLastError());
   c->steps_run += count;
   for (int o = 0; svOut && o < c->ports.nsv; ++o)
     HIP_TRY(hipMemcpyAsync(svOut + (size_t)o * count, c->ports.vtr + (size_t)o * c->ports.steps + tsStart,
@@ -885,6 +965,24 @@ int fdtd_hip_directivity(FdtdHipContext *c, const double *freqHz, const double *
   return FDTD_HIP_OK;
 }
 
+int fdtd_hip_set_ntff_mode(FdtdHipContext *c, int mode) {
+  if (!c) return fail("set_ntff_mode: null context");
+  if (mode < FDTD_HIP_NTFF_AUTO || mode > FDTD_HIP_NTFF_BLOCKED) return fail("set_ntff_mode: unknown mode %d", mode);
+  const bool was = ntff_blocked(c);
+  // Rows gathered in blocked mode are folded in before the switch.
+  if (was && c->ntff_buf && (c->fp32() ? ntff_flush<float>(c) : ntff_flush<double>(c))) return FDTD_HIP_ERR;
+  c->ntff_mode = mode;
+  if (ntff_blocked(c) && !was) c->blk_rows = c->blk_flushed = 0, c->blk_t0 = -1;
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return FDTD_HIP_OK;
+}
+
+int fdtd_hip_get_ntff_mode(FdtdHipContext *c, int *blocked) {
+  if (!c || !blocked) return fail("get_ntff_mode: null argument");
+  *blocked = ntff_blocked(c) ? 1 : 0;
+  return FDTD_HIP_OK;
+}
+
 static int check_tuning(const FdtdHipContext *c, const FdtdHipTuning *t, const char *who) {
   if (t->planes_per_workgroup < 0 || t->planes_per_workgroup > 4096)
     return fail("%s: %d planes per workgroup out of range", who, t->planes_per_workgroup);
@@ -923,6 +1021,8 @@ static int reset_state(FdtdHipContext *c) {
     HIP_TRY(hipMemsetAsync(c->ff_buf, 0, total * sizeof(double), c->stream));
   }
   if (c->dft_acc) HIP_TRY(hipMemsetAsync(c->dft_acc, 0, 2 * sizeof(double) * std::max(1, c->ports.nsv + c->ports.nsi) * c->ndft, c->stream));
+  if (c->ntff_buf) HIP_TRY(hipMemsetAsync(c->ntff_buf, 0, (size_t)NTFF_K * c->ntff_spad * c->esz, c->stream));
+  c->blk_t0 = c->blk_rows = c->blk_flushed = 0;
   if (c->traces_ready) {
     HIP_TRY(hipMemsetAsync(c->ports.vtr, 0, sizeof(double) * std::max(1, c->ports.nsv) * c->ports.steps, c->stream));
     HIP_TRY(hipMemsetAsync(c->ports.itr, 0, sizeof(double) * std::max(1, c->ports.nsi) * c->ports.steps, c->stream));
@@ -939,6 +1039,7 @@ int fdtd_hip_autotune(FdtdHipContext *c, int steps_per_candidate, FdtdHipTuning 
   if (ensure_traces(c) || build_coefficient_tables(c)) return FDTD_HIP_ERR;
   c->cpml_h = cpml_boxes(c, 0);
   c->cpml_e = cpml_boxes(c, 1);
+  if (ntff_blocked(c) && ensure_ntff_buffer(c)) return FDTD_HIP_ERR;
   const FdtdHipTuning saved = c->tune;
   const int saved_timing = c->timing;
   std::vector<FdtdHipTuning> cands;
@@ -1013,6 +1114,7 @@ void fdtd_hip_destroy(FdtdHipContext *c) {
   for (void *p : c->coef_tab) (void)hipFree(p);
   (void)hipFree(c->coef_idx);
   (void)hipFree(c->ff_buf);
+  (void)hipFree(c->ntff_buf);
   (void)hipFree(c->fields);
   (void)hipFree(c->coeffs);
   for (auto &e : c->ev) if (e) (void)hipEventDestroy(e);

@@ -18,6 +18,7 @@ CSRC = os.path.join(PKG_DIR, "csrc")
 
 FP64, FP32 = 0, 1
 BOUNDARY_REFERENCE, BOUNDARY_MATLAB = 0, 1
+NTFF_AUTO, NTFF_DIRECT, NTFF_BLOCKED = 0, 1, 2
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-fPIC", "-shared", "-std=c++17",
@@ -49,6 +50,7 @@ EXPORTS = [
     "fdtd_hip_upload_svoltage", "fdtd_hip_upload_scurrent", "fdtd_hip_upload_farfield",
     "fdtd_hip_upload_port_dft", "fdtd_hip_run_batch", "fdtd_hip_read_field", "fdtd_hip_write_field",
     "fdtd_hip_read_psi", "fdtd_hip_write_psi", "fdtd_hip_read_farfield", "fdtd_hip_read_port_dft", "fdtd_hip_radiated_power", "fdtd_hip_directivity",
+    "fdtd_hip_set_ntff_mode", "fdtd_hip_get_ntff_mode",
     "fdtd_hip_set_tuning", "fdtd_hip_get_tuning", "fdtd_hip_autotune",
     "fdtd_hip_get_coefficient_storage", "fdtd_hip_set_timing", "fdtd_hip_get_kernel_times", "fdtd_hip_synchronize", "fdtd_hip_destroy",
 ]
@@ -101,6 +103,8 @@ def lib():
         L.fdtd_hip_read_port_dft.argtypes = [P, D, D, LL]
         L.fdtd_hip_radiated_power.argtypes = [P, D, I]
         L.fdtd_hip_directivity.argtypes = [P, D, D, I, D, D, D, D]
+        L.fdtd_hip_set_ntff_mode.argtypes = [P, I]
+        L.fdtd_hip_get_ntff_mode.argtypes = [P, IP]
         L.fdtd_hip_set_tuning.argtypes = [P, C.POINTER(FdtdHipTuning)]
         L.fdtd_hip_get_tuning.argtypes = [P, C.POINTER(FdtdHipTuning)]
         L.fdtd_hip_autotune.argtypes = [P, I, C.POINTER(FdtdHipTuning)]

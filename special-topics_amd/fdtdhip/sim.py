@@ -203,6 +203,16 @@ class HipSim:
         native.check(self.L.fdtd_hip_get_coefficient_storage(self.h, C.byref(mode), n), "get_coefficient_storage")
         return mode.value, list(n)
 
+    def set_ntff_mode(self, mode: str):
+        """'auto' | 'direct' | 'blocked' (matrix-core K-blocked NTFF DFT)."""
+        m = {"auto": native.NTFF_AUTO, "direct": native.NTFF_DIRECT, "blocked": native.NTFF_BLOCKED}[mode]
+        native.check(self.L.fdtd_hip_set_ntff_mode(self.h, m), "set_ntff_mode")
+
+    def ntff_blocked(self) -> bool:
+        b = C.c_int(0)
+        native.check(self.L.fdtd_hip_get_ntff_mode(self.h, C.byref(b)), "get_ntff_mode")
+        return bool(b.value)
+
     def set_timing(self, enable: bool):
         native.check(self.L.fdtd_hip_set_timing(self.h, 1 if enable else 0), "set_timing")
 

@@ -150,6 +150,74 @@ def test_ifa_fp64_full_run_matches_reference(get_problem, scenario):
     sim.close()
 
 
+@pytest.mark.parametrize("scenario", ["ifa-matlab-7000", "ifa-config1-7000"])
+def test_ghost_plane_drives_the_late_time_growth(get_problem, scenario):
+    """The reference's 7000-step IFA regrows from about step 5000; with the i = nx ghost
+    plane skipped (boundary=matlab) the same run decays.  The growth is therefore the
+    ghost plane's, not a CPML or source instability."""
+    prob = get_problem(scenario)
+    ref_v = G.f64(G.load(f"run-{scenario}.json")["v"][0])
+    sim = _sim(prob, boundary="matlab")
+    v, _ = sim.run(0, prob.steps)
+    sim.close()
+    win = lambda x, a: np.max(np.abs(x[a:a + 500]))
+    assert np.all(np.isfinite(v))
+    assert win(ref_v, 6500) > win(ref_v, 4500) * 100           # reference: growing
+    assert win(v[0], 6500) < win(v[0], 4500) * 0.5               # matlab: still decaying
+    assert win(v[0], 6500) < 1e-3 * win(v[0], 0)
+
+
+def _ntff_runs(prob, precision, batches):
+    out = {}
+    for mode in ("direct", "blocked"):
+        sim = _sim(prob, precision=precision)
+        sim.set_ntff_mode(mode)
+        assert sim.ntff_blocked() == (mode == "blocked")
+        for s, n in batches:
+            sim.run(s, n)
+        out[mode] = [sim.farfield(q) for q in range(24)]
+        sim.close()
+    return out
+
+
+def _assert_ntff_close(runs, rel):
+    for (dr, di), (br, bi) in zip(runs["direct"], runs["blocked"]):
+        scale = max(np.max(np.abs(dr)), np.max(np.abs(di)))
+        assert scale > 0
+        assert np.max(np.abs(br - dr)) <= rel * scale and np.max(np.abs(bi - di)) <= rel * scale
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_blocked_ntff_matches_direct(get_problem, precision):
+    """The matrix-core K-blocked NTFF DFT equals the step-by-step accumulation to FP64
+    rounding, across partial blocks at uneven batch boundaries."""
+    prob = get_problem("golden-small")
+    runs = _ntff_runs(prob, precision, ((0, 37), (37, 64), (101, 59)))
+    _assert_ntff_close(runs, 1e-12)
+
+
+def test_blocked_ntff_many_frequencies(get_problem, tmp_path):
+    """20 NTFF frequencies: one launch for the first 16 (two 16-column phase tiles) and
+    one for the remaining 4."""
+    import json
+    import os
+    scen = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "scenarios", "golden-small.json")))
+    scen["farfield"]["frequencies"] = [1e9 * (1 + 0.5 * q) for q in range(20)]
+    path = tmp_path / "golden-small-ff20.json"
+    path.write_text(json.dumps(scen))
+    prob = get_problem(str(path))
+    assert prob.header["ff"]["nFreq"] == 20
+    _assert_ntff_close(_ntff_runs(prob, "fp64", ((0, 150),)), 1e-12)
+
+
+def test_fp32_defaults_to_blocked_ntff(get_problem):
+    prob = get_problem("golden-small")
+    a, b = _sim(prob, precision="fp32"), _sim(prob)
+    assert a.ntff_blocked() and not b.ntff_blocked()
+    a.close()
+    b.close()
+
+
 def test_port_running_dft_matches_trace_dft(get_problem):
     prob = get_problem("golden-small")
     freqs = np.arange(1, 41) * 0.5e9
