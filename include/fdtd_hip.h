@@ -178,6 +178,12 @@ int fdtd_hip_set_ntff_mode(FdtdHipContext *ctx, int mode);
 /* *blocked = 1 when the context accumulates the NTFF DFT in blocked form. */
 int fdtd_hip_get_ntff_mode(FdtdHipContext *ctx, int *blocked);
 
+/* Whole 64-step chunks of a batch are captured once into a HIP graph and replayed
+ * (one submission per 64 steps instead of six per step); enable = 0 launches every
+ * kernel directly.  Results are identical either way.  Default: enabled.  No
+ * counterpart in the reference ABI. */
+int fdtd_hip_set_graphs(FdtdHipContext *ctx, int enable);
+
 /* Launch shape and coefficient storage of the stencil kernels.  Every setting gives
  * bit-identical results; only speed and memory differ.  Zero fields select the
  * defaults. */

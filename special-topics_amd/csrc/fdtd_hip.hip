@@ -98,6 +98,13 @@ struct FdtdHipContext {
   int blk_rows = 0;        // rows gathered into the current block
   int blk_flushed = 0;     // rows already folded into the accumulators
 
+  // Captured graph of GRAPH_STEPS steps, replayed by run_batch; its kernels take their
+  // time step from *d_step (set before a run of replays, advanced by the graph itself).
+  int use_graphs = 1;
+  int *d_step = nullptr;
+  hipGraphExec_t graph = nullptr;
+  int graph_key[6] = {-1, -1, -1, -1, -1, -1};
+
   FdtdHipTuning tune{0, 0, 0};  // requested launch shape / coefficient storage (0 = default)
   int coef_mode = -1;           // CoefMode in use; -1 = not decided yet
   int coef_built = 0;           // deduplicated tables built (idx width in coef_idx_bytes)
@@ -138,6 +145,13 @@ struct FdtdHipContext {
 };
 
 namespace {
+
+// A captured graph bakes in every pointer and launch parameter of a step: any upload,
+// retuning or coefficient rebuild drops it, and the next batch captures a new one.
+void drop_graph(FdtdHipContext *c) {
+  if (c->graph) (void)hipGraphExecDestroy(c->graph);
+  c->graph = nullptr;
+}
 
 template <typename T>
 Grid<T> make_grid(const FdtdHipContext *c) {
@@ -213,6 +227,7 @@ int upload_vec(FdtdHipContext *c, void *dst, const std::vector<T> &v) {
 template <typename S>
 int upload_coeff_impl(FdtdHipContext *c, int which, const S *host, long long n) {
   if (!c) return fail("upload_coeff: null context");
+  drop_graph(c);
   if (which < 0 || which >= NCOEFF) return fail("upload_coeff: coefficient id %d out of range", which);
   const int comp = kCoeffOwner[which];
   if (n != c->comp_len(comp))
@@ -457,28 +472,35 @@ NtffBlock<T> ntff_block(const FdtdHipContext *c) {
   return nb;
 }
 
-// Fold rows [blk_flushed, blk_rows) of the current block into the accumulators.
+// Fold rows [r0, r1) of the block whose first step is step_of(tsp, ts0).
 template <typename T>
-int ntff_flush(FdtdHipContext *c) {
-  if (c->blk_rows <= c->blk_flushed) return FDTD_HIP_OK;
+void ntff_fold(FdtdHipContext *c, const int *tsp, int ts0, int r0, int r1) {
   const NtffBlock<T> nb = ntff_block<T>(c);
   // Persistent grid: a few workgroups per CU, each looping over sample tiles.
   const unsigned grid = (unsigned)std::min<long>((c->ntff_spad / 16 + 3) / 4, 2048);
   for (int f0 = 0; f0 < c->nff; f0 += NTFF_FREQ_GROUP) {
     if (2 * (c->nff - f0) <= 16)
-      hipLaunchKernelGGL((k_ntff_mfma<T, 1>), dim3(grid), dim3(256), 0, c->stream, c->ff, nb, c->blk_t0,
-                         c->blk_flushed, c->blk_rows, f0);
+      hipLaunchKernelGGL((k_ntff_mfma<T, 1>), dim3(grid), dim3(256), 0, c->stream, c->ff, nb, tsp, ts0, r0, r1, f0);
     else
-      hipLaunchKernelGGL((k_ntff_mfma<T, 2>), dim3(grid), dim3(256), 0, c->stream, c->ff, nb, c->blk_t0,
-                         c->blk_flushed, c->blk_rows, f0);
+      hipLaunchKernelGGL((k_ntff_mfma<T, 2>), dim3(grid), dim3(256), 0, c->stream, c->ff, nb, tsp, ts0, r0, r1, f0);
   }
+}
+
+// Fold rows [blk_flushed, blk_rows) of the current block into the accumulators.
+template <typename T>
+int ntff_flush(FdtdHipContext *c) {
+  if (c->blk_rows <= c->blk_flushed) return FDTD_HIP_OK;
+  ntff_fold<T>(c, nullptr, c->blk_t0, c->blk_flushed, c->blk_rows);
   c->blk_flushed = c->blk_rows;
   if (c->blk_rows == NTFF_K) c->blk_t0 += NTFF_K, c->blk_rows = c->blk_flushed = 0;
   return FDTD_HIP_OK;
 }
 
+// The launches of one time step.  Direct: tsp == nullptr and ts is the step.  Under
+// graph capture: ts is the step's offset within the graph and *tsp the graph's first
+// step; graphs start on a 64-step NTFF block, so offset = block row.
 template <typename T>
-int launch_step(FdtdHipContext *c, int ts) {
+int launch_step(FdtdHipContext *c, int ts, const int *tsp = nullptr) {
   const Grid<T> g = make_grid<T>(c);
   const FaceSet<T> fs = make_faces<T>(c);
   const dim3 block(256);
@@ -502,12 +524,16 @@ int launch_step(FdtdHipContext *c, int ts) {
   if (c->ports.nvs + c->ports.nsv + c->ports.nsi > 0) {
     PortState ps = c->ports;
     ps.ndft = c->ndft; ps.w = c->dft_w; ps.dft = c->dft_acc;
-    hipLaunchKernelGGL(k_port_epilogue<T>, dim3(1), block, 0, c->stream, g, ps, ts);
+    hipLaunchKernelGGL(k_port_epilogue<T>, dim3(1), block, 0, c->stream, g, ps, tsp, ts);
   }
   if (t) HIP_TRY(hipEventRecord(c->ev[5], c->stream));
   if (c->nff > 0) {
     const long total = 2 * ((long)c->ff.nj * c->ff.nk + (long)c->ff.ni * c->ff.nk + (long)c->ff.ni * c->ff.nj);
-    if (ntff_blocked(c)) {
+    if (ntff_blocked(c) && tsp) {
+      hipLaunchKernelGGL(k_ntff_gather<T>, dim3((unsigned)((total + 255) / 256)), block, 0, c->stream, g, c->ff,
+                         ntff_block<T>(c), ts);
+      if (ts == NTFF_K - 1) ntff_fold<T>(c, tsp, 0, 0, NTFF_K);
+    } else if (ntff_blocked(c)) {
       // Steps must follow on from the block; a jump (a new run) starts a new block.
       if (ts != c->blk_t0 + c->blk_rows) {
         if (ntff_flush<T>(c)) return FDTD_HIP_ERR;
@@ -519,7 +545,7 @@ int launch_step(FdtdHipContext *c, int ts) {
       ++c->blk_rows;
       if (c->blk_rows == NTFF_K && ntff_flush<T>(c)) return FDTD_HIP_ERR;
     } else {
-      hipLaunchKernelGGL(k_ntff_direct<T>, dim3((unsigned)((total + 255) / 256)), block, 0, c->stream, g, c->ff, ts);
+      hipLaunchKernelGGL(k_ntff_direct<T>, dim3((unsigned)((total + 255) / 256)), block, 0, c->stream, g, c->ff, tsp, ts);
     }
   }
   if (t) HIP_TRY(hipEventRecord(c->ev[6], c->stream));
@@ -533,6 +559,63 @@ int launch_step(FdtdHipContext *c, int ts) {
       HIP_TRY(hipEventElapsedTime(&ms, c->ev[q], c->ev[q + 1]));
       c->times.ms[slot[q]] += ms;
       c->times.launches[slot[q]] += 1;
+    }
+  }
+  return FDTD_HIP_OK;
+}
+
+constexpr int GRAPH_STEPS = NTFF_K;
+
+// Everything a captured step depends on besides the device data itself.
+void graph_key(const FdtdHipContext *c, int key[6]) {
+  key[0] = c->tune.planes_per_workgroup; key[1] = c->tune.cells_per_thread;
+  key[2] = c->tune.coefficient_storage; key[3] = c->coef_mode;
+  key[4] = ntff_blocked(c) ? 1 : 0; key[5] = c->ndft;
+}
+
+// Capture GRAPH_STEPS steps plus the step advance into an executable graph.
+template <typename T>
+int build_graph(FdtdHipContext *c) {
+  drop_graph(c);
+  HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  int rc = FDTD_HIP_OK;
+  for (int k = 0; k < GRAPH_STEPS && !rc; ++k) rc = launch_step<T>(c, k, c->d_step);
+  if (!rc) hipLaunchKernelGGL(k_advance_step, dim3(1), dim3(64), 0, c->stream, c->d_step, GRAPH_STEPS);
+  hipGraph_t g = nullptr;
+  const hipError_t end = hipStreamEndCapture(c->stream, &g);
+  if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
+  if (end != hipSuccess) return fail("graph capture failed: %s", hipGetErrorString(end));
+  const hipError_t inst = hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (inst != hipSuccess) { c->graph = nullptr; return fail("graph instantiation failed: %s", hipGetErrorString(inst)); }
+  graph_key(c, c->graph_key);
+  return FDTD_HIP_OK;
+}
+
+// Steps [ts, ts + count): whole 64-step graphs where the NTFF block state allows,
+// direct launches for the rest.
+template <typename T>
+int run_steps(FdtdHipContext *c, int ts, int count) {
+  bool base_set = false;
+  for (int s = 0; s < count;) {
+    const int t = ts + s;
+    const bool aligned = !ntff_blocked(c) || (c->blk_rows == 0 && c->blk_t0 == t);
+    if (c->use_graphs && !c->timing && count - s >= GRAPH_STEPS && aligned) {
+      int key[6];
+      graph_key(c, key);
+      if (!c->graph || std::memcmp(key, c->graph_key, sizeof key) != 0) {
+        if (build_graph<T>(c)) return FDTD_HIP_ERR;
+        base_set = false;
+      }
+      if (!base_set) hipLaunchKernelGGL(k_set_step, dim3(1), dim3(64), 0, c->stream, c->d_step, t);
+      base_set = true;
+      HIP_TRY(hipGraphLaunch(c->graph, c->stream));
+      if (ntff_blocked(c)) c->blk_t0 = t + GRAPH_STEPS;
+      s += GRAPH_STEPS;
+    } else {
+      if (launch_step<T>(c, t)) return FDTD_HIP_ERR;
+      base_set = false;
+      ++s;
     }
   }
   return FDTD_HIP_OK;
@@ -587,6 +670,7 @@ int fdtd_hip_init(const FdtdHipDims *dims, FdtdHipContext **out) {
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) rc = fail("init: stream creation failed");
   if (!rc) rc = dev_alloc(c, &c->fields, 6 * c->stride_bytes);
   if (!rc) rc = dev_alloc(c, &c->coeffs, NCOEFF * c->stride_bytes);
+  if (!rc) rc = dev_alloc(c, (void **)&c->d_step, 64);
   for (int q = 0; !rc && q < 8; ++q)
     if (hipEventCreate(&c->ev[q]) != hipSuccess) rc = fail("init: event creation failed");
   if (!rc && hipStreamSynchronize(c->stream) != hipSuccess) rc = fail("init: synchronize failed");
@@ -614,6 +698,7 @@ int fdtd_hip_upload_cpml(FdtdHipContext *c, int faceId, int nc, const double *b_
                          const double *CPsi1, long long n1, const double *CPsi2, long long n2,
                          const double *CPsi3, long long n3, int m_start, int e_start) {
   if (!c) return fail("upload_cpml: null context");
+  drop_graph(c);
   if (faceId < 0 || faceId > 5) return fail("upload_cpml: face id %d out of range", faceId);
   const int axis = faceId / 2;
   const int n = axis == 0 ? c->d.nx : (axis == 1 ? c->d.ny : c->d.nz);
@@ -666,6 +751,7 @@ int fdtd_hip_upload_cpml(FdtdHipContext *c, int faceId, int nc, const double *b_
 int fdtd_hip_upload_vsource(FdtdHipContext *c, int dir, const int *idx, const double *coef, int count,
                             const double *waveform, int nSteps) {
   if (!c) return fail("upload_vsource: null context");
+  drop_graph(c);
   if (dir < 0 || dir > 2) return fail("upload_vsource: direction %d out of range", dir);
   if (c->ports.nvs >= MAX_PORT_OBJS) return fail("upload_vsource: at most %d sources", MAX_PORT_OBJS);
   if (count < 0 || (count > 0 && (!idx || !coef))) return fail("upload_vsource: bad edge list");
@@ -692,6 +778,7 @@ int fdtd_hip_upload_vsource(FdtdHipContext *c, int dir, const int *idx, const do
 
 int fdtd_hip_upload_svoltage(FdtdHipContext *c, int dir, const int *idx, int count, double Csvf) {
   if (!c) return fail("upload_svoltage: null context");
+  drop_graph(c);
   if (dir < 0 || dir > 2) return fail("upload_svoltage: direction %d out of range", dir);
   if (c->ports.nsv >= MAX_PORT_OBJS) return fail("upload_svoltage: at most %d observers", MAX_PORT_OBJS);
   if (c->traces_ready) return fail("upload_svoltage: observers must be uploaded before the first step");
@@ -713,6 +800,7 @@ int fdtd_hip_upload_svoltage(FdtdHipContext *c, int dir, const int *idx, int cou
 int fdtd_hip_upload_scurrent(FdtdHipContext *c, const int *comp, const int *index, const double *weight,
                              int count, int negate) {
   if (!c) return fail("upload_scurrent: null context");
+  drop_graph(c);
   if (c->ports.nsi >= MAX_PORT_OBJS) return fail("upload_scurrent: at most %d observers", MAX_PORT_OBJS);
   if (c->traces_ready) return fail("upload_scurrent: observers must be uploaded before the first step");
   if (count < 0 || (count > 0 && (!comp || !index || !weight))) return fail("upload_scurrent: bad term list");
@@ -734,6 +822,7 @@ int fdtd_hip_upload_scurrent(FdtdHipContext *c, const int *comp, const int *inde
 
 int fdtd_hip_upload_farfield(FdtdHipContext *c, int nFreq, const int *box, const double *phase, int nSteps) {
   if (!c) return fail("upload_farfield: null context");
+  drop_graph(c);
   if (c->nff) return fail("upload_farfield: already uploaded");
   if (nFreq <= 0 || !box || !phase) return fail("upload_farfield: bad arguments");
   if (nSteps < c->d.numberOfTimeSteps) return fail("upload_farfield: phase table has %d steps, need %d", nSteps, c->d.numberOfTimeSteps);
@@ -763,6 +852,7 @@ int fdtd_hip_upload_farfield(FdtdHipContext *c, int nFreq, const int *box, const
 
 int fdtd_hip_upload_port_dft(FdtdHipContext *c, int nFreq, const double *freqHz) {
   if (!c) return fail("upload_port_dft: null context");
+  drop_graph(c);
   if (c->ndft) return fail("upload_port_dft: already uploaded");
   if (nFreq <= 0 || !freqHz) return fail("upload_port_dft: bad arguments");
   std::vector<double> w(nFreq);
@@ -783,10 +873,7 @@ int fdtd_hip_run_batch(FdtdHipContext *c, int tsStart, int count, double *svOut,
   c->cpml_h = cpml_boxes(c, 0);
   c->cpml_e = cpml_boxes(c, 1);
   if (ntff_blocked(c) && ensure_ntff_buffer(c)) return FDTD_HIP_ERR;
-  for (int s = 0; s < count; ++s) {
-    const int rc = c->fp32() ? launch_step<float>(c, tsStart + s) : launch_step<double>(c, tsStart + s);
-    if (rc) return rc;
-  }
+  if (c->fp32() ? run_steps<float>(c, tsStart, count) : run_steps<double>(c, tsStart, count)) return FDTD_HIP_ERR;
   // Leave the accumulators complete at the end of every batch.
   if (ntff_blocked(c) && (c->fp32() ? ntff_flush<float>(c) : ntff_flush<double>(c))) return FDTD_HIP_ERR;
   HIP_TRY(hipGetLastError());
@@ -967,6 +1054,7 @@ int fdtd_hip_directivity(FdtdHipContext *c, const double *freqHz, const double *
 
 int fdtd_hip_set_ntff_mode(FdtdHipContext *c, int mode) {
   if (!c) return fail("set_ntff_mode: null context");
+  drop_graph(c);
   if (mode < FDTD_HIP_NTFF_AUTO || mode > FDTD_HIP_NTFF_BLOCKED) return fail("set_ntff_mode: unknown mode %d", mode);
   const bool was = ntff_blocked(c);
   // Rows gathered in blocked mode are folded in before the switch.
@@ -974,6 +1062,13 @@ int fdtd_hip_set_ntff_mode(FdtdHipContext *c, int mode) {
   c->ntff_mode = mode;
   if (ntff_blocked(c) && !was) c->blk_rows = c->blk_flushed = 0, c->blk_t0 = -1;
   HIP_TRY(hipStreamSynchronize(c->stream));
+  return FDTD_HIP_OK;
+}
+
+int fdtd_hip_set_graphs(FdtdHipContext *c, int enable) {
+  if (!c) return fail("set_graphs: null context");
+  drop_graph(c);
+  c->use_graphs = enable ? 1 : 0;
   return FDTD_HIP_OK;
 }
 
@@ -997,6 +1092,7 @@ static int check_tuning(const FdtdHipContext *c, const FdtdHipTuning *t, const c
 
 int fdtd_hip_set_tuning(FdtdHipContext *c, const FdtdHipTuning *t) {
   if (!c || !t) return fail("set_tuning: null argument");
+  drop_graph(c);
   if (check_tuning(c, t, "set_tuning")) return FDTD_HIP_ERR;
   if (t->coefficient_storage != c->tune.coefficient_storage) c->coef_mode = -1;
   c->tune = *t;
@@ -1033,6 +1129,7 @@ static int reset_state(FdtdHipContext *c) {
 
 int fdtd_hip_autotune(FdtdHipContext *c, int steps_per_candidate, FdtdHipTuning *chosen) {
   if (!c) return fail("autotune: null context");
+  drop_graph(c);
   if (c->steps_run) return fail("autotune: must run before the first time step (it resets the state)");
   if (steps_per_candidate < 1) steps_per_candidate = 2;
   const int steps = std::min(steps_per_candidate + 1, c->d.numberOfTimeSteps);
@@ -1115,6 +1212,8 @@ void fdtd_hip_destroy(FdtdHipContext *c) {
   (void)hipFree(c->coef_idx);
   (void)hipFree(c->ff_buf);
   (void)hipFree(c->ntff_buf);
+  drop_graph(c);
+  (void)hipFree(c->d_step);
   (void)hipFree(c->fields);
   (void)hipFree(c->coeffs);
   for (auto &e : c->ev) if (e) (void)hipEventDestroy(e);

@@ -515,11 +515,16 @@ __device__ __forceinline__ double ordered_wave_sum(double acc, double mine, int 
   return acc;
 }
 
+// Time step of a launch: `off` alone when launched directly, or the device-held step
+// of the first step of a replayed graph plus the kernel's offset within the graph.
+__device__ __forceinline__ int step_of(const int *tsp, int off) { return off + (tsp ? *tsp : 0); }
+
 // Source injection, then port voltages and currents, then their running DFTs.
 // One block; the injection precedes the captures as in fdtdSolver.js:111-117.
 template <typename T>
-__global__ void __launch_bounds__(256) k_port_epilogue(Grid<T> g, PortState ps, int ts) {
+__global__ void __launch_bounds__(256) k_port_epilogue(Grid<T> g, PortState ps, const int *tsp, int off) {
 #pragma clang fp contract(off)
+  const int ts = step_of(tsp, off);
   __shared__ double s_val[2 * MAX_PORT_OBJS];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nwaves = blockDim.x >> 6;
   T *E[3] = {g.Ex, g.Ey, g.Ez};
@@ -597,8 +602,9 @@ __host__ __device__ __forceinline__ long ff_offset(int a, int nfreq, long sx, lo
 // all frequencies in a loop.  Face and component mapping follow farfield.js:115-246
 // (including the zp-face Jx subtract-then-add sequence, :210-226).
 template <typename T>
-__global__ void __launch_bounds__(256) k_ntff_direct(Grid<T> g, FarfieldState fs, int ts) {
+__global__ void __launch_bounds__(256) k_ntff_direct(Grid<T> g, FarfieldState fs, const int *tsp, int off) {
 #pragma clang fp contract(off)
+  const int ts = step_of(tsp, off);
   const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
   const long sx = (long)fs.nj * fs.nk, sy = (long)fs.ni * fs.nk, sz = (long)fs.ni * fs.nj;
   const long total = 2 * (sx + sy + sz);
@@ -682,6 +688,16 @@ __global__ void __launch_bounds__(256) k_ntff_direct(Grid<T> g, FarfieldState fs
     }
     rJ1[pos] = rJ1[pos] + ch * j1; iJ1[pos] = iJ1[pos] + sh * j1;
   }
+}
+
+// Advances the device-held step by `by` at the end of a replayed graph.
+__global__ void k_advance_step(int *tsp, int by) {
+  if (threadIdx.x == 0) *tsp += by;
+}
+
+// Sets the device-held step before a run of graph replays.
+__global__ void k_set_step(int *tsp, int v) {
+  if (threadIdx.x == 0) *tsp = v;
 }
 
 }  // namespace fdtd

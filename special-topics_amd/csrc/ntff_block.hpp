@@ -84,13 +84,15 @@ __global__ void __launch_bounds__(256) k_ntff_gather(Grid<T> g, FarfieldState fs
   r[nb.base[a0 + 3] + cell] = tM1;
 }
 
-// Fold rows [r0, r1) of the block starting at step ts0 into the accumulators of
+// Fold rows [r0, r1) of the block starting at step ts0 (see step_of) into the accumulators of
 // frequencies [f0, f0 + NT*8).  A wavefront handles 16-sample tiles (tiles never
 // straddle accumulators); P in LDS (NT = 2: 32 KiB), zero outside [r0, r1) and beyond
 // the last frequency.
 template <typename T, int NT>
-__global__ void __launch_bounds__(256) k_ntff_mfma(FarfieldState fs, NtffBlock<T> nb, int ts0, int r0, int r1, int f0) {
+__global__ void __launch_bounds__(256) k_ntff_mfma(FarfieldState fs, NtffBlock<T> nb, const int *tsp, int ts_off,
+                                                   int r0, int r1, int f0) {
   constexpr int NP = NT * 16;
+  const int ts0 = step_of(tsp, ts_off);
   __shared__ double P[2 * NTFF_K * NP];   // J (H phase) then M (E phase), [k][column]
   const int ncol = 2 * (fs.nfreq - f0);    // columns left from f0 (only the first NP used)
   for (int q = threadIdx.x; q < 2 * NTFF_K * NP; q += blockDim.x) {

@@ -50,7 +50,7 @@ EXPORTS = [
     "fdtd_hip_upload_svoltage", "fdtd_hip_upload_scurrent", "fdtd_hip_upload_farfield",
     "fdtd_hip_upload_port_dft", "fdtd_hip_run_batch", "fdtd_hip_read_field", "fdtd_hip_write_field",
     "fdtd_hip_read_psi", "fdtd_hip_write_psi", "fdtd_hip_read_farfield", "fdtd_hip_read_port_dft", "fdtd_hip_radiated_power", "fdtd_hip_directivity",
-    "fdtd_hip_set_ntff_mode", "fdtd_hip_get_ntff_mode",
+    "fdtd_hip_set_ntff_mode", "fdtd_hip_get_ntff_mode", "fdtd_hip_set_graphs",
     "fdtd_hip_set_tuning", "fdtd_hip_get_tuning", "fdtd_hip_autotune",
     "fdtd_hip_get_coefficient_storage", "fdtd_hip_set_timing", "fdtd_hip_get_kernel_times", "fdtd_hip_synchronize", "fdtd_hip_destroy",
 ]
@@ -105,6 +105,7 @@ def lib():
         L.fdtd_hip_directivity.argtypes = [P, D, D, I, D, D, D, D]
         L.fdtd_hip_set_ntff_mode.argtypes = [P, I]
         L.fdtd_hip_get_ntff_mode.argtypes = [P, IP]
+        L.fdtd_hip_set_graphs.argtypes = [P, I]
         L.fdtd_hip_set_tuning.argtypes = [P, C.POINTER(FdtdHipTuning)]
         L.fdtd_hip_get_tuning.argtypes = [P, C.POINTER(FdtdHipTuning)]
         L.fdtd_hip_autotune.argtypes = [P, I, C.POINTER(FdtdHipTuning)]

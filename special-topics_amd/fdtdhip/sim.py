@@ -213,6 +213,10 @@ class HipSim:
         native.check(self.L.fdtd_hip_get_ntff_mode(self.h, C.byref(b)), "get_ntff_mode")
         return bool(b.value)
 
+    def set_graphs(self, enable: bool):
+        """Replay 64-step chunks from a captured HIP graph (default) or launch directly."""
+        native.check(self.L.fdtd_hip_set_graphs(self.h, 1 if enable else 0), "set_graphs")
+
     def set_timing(self, enable: bool):
         native.check(self.L.fdtd_hip_set_timing(self.h, 1 if enable else 0), "set_timing")
 
