@@ -161,6 +161,41 @@ def ifa_study(workdir, scenario="ifa-config1-7000"):
     return out
 
 
+def sweep_study(workdir, count=64, inflight=8, seed=4):
+    """Config 4: `count` seeded IFA variants (uneven grids), FP32, every context's steps
+    replayed from captured graphs, `inflight` contexts on the GPU at once."""
+    sys.path.insert(0, os.path.join(REPO, "special-topics_amd", "scenarios"))
+    import gen_ifa_variants
+    from fdtdhip.sweep import Sweep, build_all
+    wd = os.path.join(workdir, "ifa-variants")
+    os.makedirs(wd, exist_ok=True)
+    paths = []
+    for i in range(count):
+        sc = gen_ifa_variants.scenario(seed, i)
+        path = os.path.join(wd, sc["name"] + ".json")
+        with open(path, "w") as f:
+            json.dump(sc, f)
+        paths.append(path)
+    t0 = time.perf_counter()
+    files = build_all(paths, wd, fp32=True, jobs=max(1, min(host_cores(), 16)))
+    build_s = time.perf_counter() - t0
+    res, secs, cells = Sweep("fp32", "matlab", inflight, tuning=(1, 4, 0), keep_traces=False).run(files)
+    dims = np.array([r["dims"] for r in res])
+    res_ghz = np.array([r["resonance_hz"] for r in res]) / 1e9
+    smin = np.array([r["s11_min_db"] for r in res])
+    dpk = np.array([r["peak_directivity_dbi"] for r in res])
+    return {
+        "workload": f"{count} seeded IFA variants (gen_ifa_variants.py seed {seed}), grids "
+                    f"{dims[:, 0].min()}x{dims[:, 1].min()}-{dims[:, 1].max()}x{dims[:, 2].min()}-{dims[:, 2].max()}, "
+                    "FP32, 7000 steps, matlab boundary, S11 at 1000 f, NTFF at 16 f (matrix-core blocked)",
+        "inflight": inflight, "build_s": round(build_s, 2), "seconds": round(secs, 3),
+        "mcell_updates_per_s": round(cells / secs / 1e6, 1), "ms_per_variant": round(secs / count * 1e3, 2),
+        "resonance_ghz_min_median_max": [round(float(x), 3) for x in (res_ghz.min(), np.median(res_ghz), res_ghz.max())],
+        "s11_min_db_median": round(float(np.median(smin)), 2),
+        "peak_directivity_dbi_median": round(float(np.median(dpk)), 2),
+    }
+
+
 def progress(msg: str):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -183,6 +218,7 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=0, help="0 = choose for ~20 s of CPU work")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-ifa", action="store_true", help="skip the IFA FP32-vs-FP64 time-to-solution study")
+    ap.add_argument("--no-sweep", action="store_true", help="skip the 64-variant IFA sweep")
     ap.add_argument("--ntff-mode", default="auto", choices=["auto", "direct", "blocked"],
                     help="NTFF running DFT: step-by-step (direct) or K-blocked on the matrix cores")
     ap.add_argument("--workdir", default=os.path.join(tempfile.gettempdir(), "fdtd-bench"))
@@ -308,6 +344,10 @@ def main():
     if not args.no_ifa:
         progress("IFA FP32-vs-FP64 study")
         line["ifa"] = ifa_study(args.workdir)
+
+    if not args.no_sweep:
+        progress("64-variant IFA sweep")
+        line["sweep"] = sweep_study(args.workdir)
 
     if not args.no_cpu_baseline:
         progress("CPU baseline")

@@ -7,6 +7,7 @@
 #include "fdtd_kernels.hpp"
 #include "ntff_post.hpp"
 #include "ntff_block.hpp"
+#include "earlystop_net.hpp"
 
 #include <hip/hip_runtime.h>
 
@@ -864,7 +865,7 @@ int fdtd_hip_upload_port_dft(FdtdHipContext *c, int nFreq, const double *freqHz)
   return FDTD_HIP_OK;
 }
 
-int fdtd_hip_run_batch(FdtdHipContext *c, int tsStart, int count, double *svOut, double *siOut) {
+int fdtd_hip_run_batch_async(FdtdHipContext *c, int tsStart, int count) {
   if (!c) return fail("run_batch: null context");
   if (tsStart < 0 || count < 0 || tsStart + count > c->d.numberOfTimeSteps)
     return fail("run_batch: steps [%d,%d) outside [0,%d)", tsStart, tsStart + count, c->d.numberOfTimeSteps);
@@ -878,6 +879,14 @@ int fdtd_hip_run_batch(FdtdHipContext *c, int tsStart, int count, double *svOut,
   if (ntff_blocked(c) && (c->fp32() ? ntff_flush<float>(c) : ntff_flush<double>(c))) return FDTD_HIP_ERR;
   HIP_TRY(hipGetLastError());
   c->steps_run += count;
+  return FDTD_HIP_OK;
+}
+
+int fdtd_hip_read_traces(FdtdHipContext *c, int tsStart, int count, double *svOut, double *siOut) {
+  if (!c) return fail("read_traces: null context");
+  if (tsStart < 0 || count < 0 || tsStart + count > c->d.numberOfTimeSteps)
+    return fail("read_traces: steps [%d,%d) outside [0,%d)", tsStart, tsStart + count, c->d.numberOfTimeSteps);
+  if (!c->traces_ready && (svOut || siOut)) return fail("read_traces: nothing has run yet");
   for (int o = 0; svOut && o < c->ports.nsv; ++o)
     HIP_TRY(hipMemcpyAsync(svOut + (size_t)o * count, c->ports.vtr + (size_t)o * c->ports.steps + tsStart,
                            count * sizeof(double), hipMemcpyDeviceToHost, c->stream));
@@ -885,6 +894,19 @@ int fdtd_hip_run_batch(FdtdHipContext *c, int tsStart, int count, double *svOut,
     HIP_TRY(hipMemcpyAsync(siOut + (size_t)o * count, c->ports.itr + (size_t)o * c->ports.steps + tsStart,
                            count * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
+  return FDTD_HIP_OK;
+}
+
+int fdtd_hip_run_batch(FdtdHipContext *c, int tsStart, int count, double *svOut, double *siOut) {
+  if (fdtd_hip_run_batch_async(c, tsStart, count)) return FDTD_HIP_ERR;
+  return fdtd_hip_read_traces(c, tsStart, count, svOut, siOut);
+}
+
+int fdtd_hip_query(FdtdHipContext *c, int *idle) {
+  if (!c || !idle) return fail("query: null argument");
+  const hipError_t e = hipStreamQuery(c->stream);
+  if (e != hipSuccess && e != hipErrorNotReady) return fail("query: %s", hipGetErrorString(e));
+  *idle = e == hipSuccess ? 1 : 0;
   return FDTD_HIP_OK;
 }
 
@@ -1219,6 +1241,99 @@ void fdtd_hip_destroy(FdtdHipContext *c) {
   for (auto &e : c->ev) if (e) (void)hipEventDestroy(e);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
+}
+
+
+// ------------------------------------------------------------ early-stop network --
+
+struct FdtdHipEarlyStop {
+  hipStream_t stream = nullptr;
+  uint16_t *w = nullptr;
+  float *b = nullptr;
+  float *act[3] = {};      // input, ping, pong
+  int cap = 0;             // batch capacity of the activation buffers
+};
+
+static long long es_weight_count() {
+  long long n = 0;
+  int cin = ES_IN_CH;
+  for (int l = 0; l < 4; ++l) { n += (long long)ES_CH * cin * ES_KW; cin = ES_CH; }
+  return n + (long long)ES_HIDDEN * ES_CH * ES_LEN[4] + (long long)ES_OUT * ES_HIDDEN;
+}
+
+static long long es_bias_count() { return 4LL * ES_CH + ES_HIDDEN + ES_OUT; }
+
+void fdtd_hip_es_destroy(FdtdHipEarlyStop *n) {
+  if (!n) return;
+  if (n->stream) (void)hipStreamSynchronize(n->stream);
+  (void)hipFree(n->w);
+  (void)hipFree(n->b);
+  for (float *a : n->act) (void)hipFree(a);
+  if (n->stream) (void)hipStreamDestroy(n->stream);
+  delete n;
+}
+
+int fdtd_hip_es_create(const unsigned short *weights, long long nweights, const float *biases, long long nbiases,
+                       FdtdHipEarlyStop **out) {
+  if (!weights || !biases || !out) return fail("es_create: null argument");
+  if (nweights != es_weight_count()) return fail("es_create: %lld weights, expected %lld", nweights, es_weight_count());
+  if (nbiases != es_bias_count()) return fail("es_create: %lld biases, expected %lld", nbiases, es_bias_count());
+  FdtdHipEarlyStop *n = new FdtdHipEarlyStop;
+  *out = nullptr;
+  if (hipStreamCreateWithFlags(&n->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipMalloc(&n->w, nweights * 2) != hipSuccess || hipMalloc(&n->b, nbiases * 4) != hipSuccess ||
+      hipMemcpy(n->w, weights, nweights * 2, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(n->b, biases, nbiases * 4, hipMemcpyHostToDevice) != hipSuccess) {
+    fdtd_hip_es_destroy(n);
+    return fail("es_create: device allocation or copy failed");
+  }
+  *out = n;
+  return FDTD_HIP_OK;
+}
+
+int fdtd_hip_es_predict(FdtdHipEarlyStop *n, int batch, const float *x, float *y) {
+  if (!n || !x || !y) return fail("es_predict: null argument");
+  if (batch <= 0) return fail("es_predict: batch %d", batch);
+  const long in_sz = (long)ES_IN_CH * ES_LEN[0];
+  const long mid_sz = std::max<long>((long)ES_CH * ES_LEN[1], ES_OUT);
+  if (batch > n->cap) {
+    for (float *&a : n->act) { (void)hipFree(a); a = nullptr; }
+    if (hipMalloc(&n->act[0], (size_t)batch * in_sz * 4) != hipSuccess ||
+        hipMalloc(&n->act[1], (size_t)batch * mid_sz * 4) != hipSuccess ||
+        hipMalloc(&n->act[2], (size_t)batch * mid_sz * 4) != hipSuccess) {
+      n->cap = 0;
+      return fail("es_predict: device allocation failed");
+    }
+    n->cap = batch;
+  }
+  HIP_TRY(hipMemcpyAsync(n->act[0], x, (size_t)batch * in_sz * 4, hipMemcpyHostToDevice, n->stream));
+  const uint16_t *w = n->w;
+  const float *b = n->b;
+  float *src = n->act[0], *dst = n->act[1];
+  int cin = ES_IN_CH;
+  for (int l = 0; l < 4; ++l) {
+    const long outs = (long)batch * ES_CH * ES_LEN[l + 1];
+    hipLaunchKernelGGL(k_es_conv, dim3((unsigned)((outs + 255) / 256)), dim3(256), 0, n->stream, src, w, b, dst,
+                       batch, cin, ES_LEN[l], ES_LEN[l + 1]);
+    w += (long)ES_CH * cin * ES_KW;
+    b += ES_CH;
+    cin = ES_CH;
+    src = dst;
+    dst = dst == n->act[1] ? n->act[2] : n->act[1];
+  }
+  const int K1 = ES_CH * ES_LEN[4];
+  hipLaunchKernelGGL(k_es_dense, dim3((unsigned)(((long)batch * ES_HIDDEN * 64 + 255) / 256)), dim3(256), 0, n->stream,
+                     src, w, b, dst, batch, K1, ES_HIDDEN, 1);
+  w += (long)ES_HIDDEN * K1;
+  b += ES_HIDDEN;
+  src = dst;
+  dst = dst == n->act[1] ? n->act[2] : n->act[1];
+  hipLaunchKernelGGL(k_es_dense, dim3((unsigned)(((long)batch * ES_OUT * 64 + 255) / 256)), dim3(256), 0, n->stream,
+                     src, w, b, dst, batch, ES_HIDDEN, ES_OUT, 0);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpyAsync(y, dst, (size_t)batch * ES_OUT * 4, hipMemcpyDeviceToHost, n->stream));
+  HIP_TRY(hipStreamSynchronize(n->stream));
+  return FDTD_HIP_OK;
 }
 
 }  // extern "C"

@@ -1,0 +1,304 @@
+"""Learned early stop of the IFA port transient.
+
+A run is stopped after N = 1500 of its 7000 steps; a small network predicts the
+remaining port-voltage and port-current tail from the first N steps, and S11 is taken
+from the measured prefix plus the predicted tail.  When the network's residual-energy
+estimate (tail energy over total) exceeds 1e-4 the full-length FP64 run is used
+instead.
+
+Representation (all per channel, V and I, each scaled by its prefix max |x|):
+  input  x: the 1500-step prefix mean-pooled by 4  -> 375 samples
+  target  : the tail, steps 1500..6995, mean-pooled by 8 -> 687 samples,
+            plus log10 of the residual-energy ratio.
+Pooled samples sit at window centres; for signals band-limited to 6 GHz the pooled
+tail reproduces the full-rate DFT sum to about 0.02 dB (midpoint rule, phase step
+2*pi*6 GHz*8*dt = 0.24 rad).
+
+    python -m fdtdhip.earlystop gen-train --first 0 --count 2000 --out train0.npz
+    python -m fdtdhip.earlystop gen-heldout --first 4000 --count 500 --out heldout.npz
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+from .sweep import LazyBuilds, Sweep, frequency_grid
+
+N_PREFIX = 1500
+POOL_IN = 4
+POOL_OUT = 8
+STEPS = 7000
+TAIL = (STEPS - N_PREFIX) // POOL_OUT * POOL_OUT     # 5496
+N_IN = N_PREFIX // POOL_IN                           # 375
+N_OUT = TAIL // POOL_OUT                             # 687
+FALLBACK_LOG_RATIO = -4.0                            # residual energy > 1e-4 of total
+
+
+def pool(x: np.ndarray, k: int) -> np.ndarray:
+    n = x.shape[-1] // k * k
+    return x[..., :n].reshape(*x.shape[:-1], -1, k).mean(axis=-1)
+
+
+def features(v: np.ndarray, i: np.ndarray):
+    """Network input [2, N_IN] and the channel scales from the first N_PREFIX steps."""
+    scale = np.array([np.max(np.abs(v[:N_PREFIX])), np.max(np.abs(i[:N_PREFIX]))])
+    scale = np.where(scale > 0, scale, 1.0)
+    x = np.stack([pool(v[:N_PREFIX] / scale[0], POOL_IN), pool(i[:N_PREFIX] / scale[1], POOL_IN)])
+    return x.astype(np.float32), scale
+
+
+def targets(v: np.ndarray, i: np.ndarray, scale: np.ndarray):
+    """Pooled normalised tail [2, N_OUT] and log10 residual-energy ratio."""
+    vn, inn = v / scale[0], i / scale[1]
+    tail = np.stack([pool(vn[N_PREFIX:N_PREFIX + TAIL], POOL_OUT), pool(inn[N_PREFIX:N_PREFIX + TAIL], POOL_OUT)])
+    e = vn ** 2 + inn ** 2
+    ratio = e[N_PREFIX:].sum() / max(e.sum(), 1e-300)
+    return tail.astype(np.float32), np.float32(np.log10(max(ratio, 1e-30)))
+
+
+def tail_dft(tail: np.ndarray, scale: np.ndarray, dt: float, freqs: np.ndarray):
+    """DFT contribution of a pooled tail (un-normalised) with the conventions of
+    js_fdtd/src/dft.js:17-38: shift 0 for V, -dt/2 for I; returns (V(f), I(f)).  Each
+    pooled sample stands for its window's sum at the window centre (midpoint rule)."""
+    ph = _tail_phases(dt, freqs)
+    return ph[0] @ (tail[0] * scale[0]), ph[1] @ (tail[1] * scale[1])
+
+
+_PHASES = {}
+
+
+def _tail_phases(dt: float, freqs: np.ndarray):
+    """dt * POOL_OUT * exp(-j w t) at the pooled-window centres, for V and for I
+    (cached: it depends on dt and the frequency grid only)."""
+    key = (float(dt), freqs.size, float(freqs[0]), float(freqs[-1]))
+    if key not in _PHASES:
+        centres = N_PREFIX + POOL_OUT * np.arange(N_OUT) + (POOL_OUT - 1) / 2
+        w = 2 * np.pi * freqs
+        _PHASES[key] = [dt * POOL_OUT * np.exp(-1j * np.outer(w, (centres + 1) * dt + shift))
+                        for shift in (0.0, -dt / 2)]
+    return _PHASES[key]
+
+
+def prefix_dft(v: np.ndarray, i: np.ndarray, dt: float, freqs: np.ndarray):
+    w = 2 * np.pi * freqs
+    n = np.arange(N_PREFIX)
+    V = dt * (np.exp(-1j * np.outer(w, (n + 1) * dt)) @ v[:N_PREFIX])
+    I = dt * (np.exp(-1j * np.outer(w, (n + 1) * dt - dt / 2)) @ i[:N_PREFIX])
+    return V, I
+
+
+def s11_from(V, I, z=50.0):
+    return (0.5 * (V - z * I)) / (0.5 * (V + z * I))
+
+
+# ------------------------------------------------------------------------ data --
+
+def _scenarios(first, count, seed, workdir, steps=STEPS):
+    here = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    sys.path.insert(0, os.path.join(here, "scenarios"))
+    import gen_ifa_variants
+    os.makedirs(workdir, exist_ok=True)
+    paths, params = [], []
+    for idx in range(first, first + count):
+        sc = gen_ifa_variants.scenario(seed, idx, steps)
+        path = os.path.join(workdir, sc["name"] + (f"-{steps}" if steps != STEPS else "") + ".json")
+        with open(path, "w") as f:
+            json.dump(sc, f)
+        paths.append(path)
+        params.append([sc["parameters"][k] for k in ("arm_length", "arm_height", "feed_short", "ground_w", "ground_l")])
+    return paths, np.array(params)
+
+
+def gen_train(first, count, out, seed=4, workdir="/tmp/es-data", inflight=8, jobs=16):
+    """FP32 full runs (matlab boundary) of variants [first, first+count): input, target."""
+    paths, params = _scenarios(first, count, seed, workdir)
+    builds = LazyBuilds(paths, workdir, fp32=True, jobs=jobs)
+    res, secs, cells = Sweep("fp32", "matlab", inflight, tuning=(1, 4, 0), keep_traces=True,
+                             farfield=False, s11=False, progress_every=100).run(builds)
+    builds.close()
+    xs, scales, tails, logr, dims, dts = [], [], [], [], [], []
+    for r in res:
+        x, sc = features(r["v"], r["i"])
+        t, lr = targets(r["v"], r["i"], sc)
+        xs.append(x); scales.append(sc); tails.append(t); logr.append(lr); dims.append(r["dims"]); dts.append(r["dt"])
+    np.savez_compressed(out, index=np.arange(first, first + count), params=params, x=np.array(xs),
+                        scale=np.array(scales), tail=np.array(tails), log_ratio=np.array(logr),
+                        dims=np.array(dims), dt=np.array(dts))
+    return {"variants": count, "run_s": round(secs, 1), "mcell_updates_per_s": round(cells / secs / 1e6, 1)}
+
+
+def gen_heldout(first, count, out, seed=4, workdir="/tmp/es-heldout", inflight=8, jobs=16):
+    """Per held-out variant: the full-length FP64 S11 (the judge) and the FP32 prefix
+    traces at full rate (what an early-stopped run sees)."""
+    paths, params = _scenarios(first, count, seed, workdir)
+    b64 = LazyBuilds(paths, os.path.join(workdir, "f64"), fp32=False, jobs=jobs)
+    res64, secs64, cells64 = Sweep("fp64", "matlab", inflight, keep_traces=False, farfield=False,
+                                   progress_every=50).run(b64)
+    b64.close()
+    b32 = LazyBuilds(paths, os.path.join(workdir, "f32"), fp32=True, jobs=jobs)
+    res32, secs32, _ = Sweep("fp32", "matlab", inflight, tuning=(1, 4, 0), keep_traces=True, farfield=False,
+                             s11=False, steps=N_PREFIX, progress_every=100).run(b32)
+    b32.close()
+    np.savez_compressed(out, index=np.arange(first, first + count), params=params,
+                        s11_fp64=np.array([r["s11"] for r in res64]),
+                        v_prefix=np.array([r["v"] for r in res32]), i_prefix=np.array([r["i"] for r in res32]),
+                        dims=np.array([r["dims"] for r in res32]), dt=np.array([r["dt"] for r in res32]))
+    return {"variants": count, "fp64_run_s": round(secs64, 1),
+            "fp64_mcell_updates_per_s": round(cells64 / secs64 / 1e6, 1), "fp32_prefix_run_s": round(secs32, 1)}
+
+
+# ------------------------------------------------------------------ inference --
+
+class Predictor:
+    """The HIP inference kernels (fdtd_hip_es_*) with weights from earlystop_net.save."""
+
+    def __init__(self, weights_path):
+        import ctypes as C
+        from . import native
+        d = np.load(weights_path)
+        self.w = np.ascontiguousarray(d["weights_bf16"], dtype=np.uint16)
+        self.b = np.ascontiguousarray(d["biases"], dtype=np.float32)
+        self.C, self.native, self.L = C, native, native.lib()
+        self.h = C.c_void_p()
+        native.check(self.L.fdtd_hip_es_create(self.w.ctypes.data_as(C.POINTER(C.c_uint16)), self.w.size,
+                                               self.b.ctypes.data_as(C.POINTER(C.c_float)), self.b.size,
+                                               C.byref(self.h)), "es_create")
+
+    def __call__(self, x: np.ndarray) -> np.ndarray:
+        C = self.C
+        x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, 2, N_IN)
+        y = np.zeros((len(x), 2 * N_OUT + 1), dtype=np.float32)
+        self.native.check(self.L.fdtd_hip_es_predict(self.h, len(x), x.ctypes.data_as(C.POINTER(C.c_float)),
+                                                     y.ctypes.data_as(C.POINTER(C.c_float))), "es_predict")
+        return y
+
+    def close(self):
+        if self.h:
+            self.L.fdtd_hip_es_destroy(self.h)
+            self.h = None
+
+
+def decode(y: np.ndarray, x: np.ndarray):
+    """Network output -> (pooled tail [2, N_OUT] in prefix-scale units, log10 ratio)."""
+    from .earlystop_net import end_amplitude
+    a = end_amplitude(x)
+    return y[: 2 * N_OUT].reshape(2, N_OUT) * a[:, None], float(y[-1])
+
+
+def evaluate(weights_path, heldout_path, band=(0.0, 6e9)):
+    """Early-stop S11 of every held-out variant against its full-length FP64 S11."""
+    d = np.load(heldout_path)
+    dt = d["dt"]
+    freqs = frequency_grid({"maxFreq": 6e9, "step": 6e6})
+    sel = (freqs >= band[0]) & (freqs <= band[1])
+    feats = [features(v, i) for v, i in zip(d["v_prefix"], d["i_prefix"])]
+    X = np.stack([f[0] for f in feats])
+    net = Predictor(weights_path)
+    Y = net(X)
+    net.close()
+    errs, fall = [], []
+    for k, (x, scale) in enumerate(feats):
+        tail, logr = decode(Y[k], x)
+        Vp, Ip = prefix_dft(d["v_prefix"][k], d["i_prefix"][k], dt[k], freqs)
+        Vt, It = tail_dft(tail, scale, dt[k], freqs)
+        s_es = s11_from(Vp + Vt, Ip + It)
+        s_64 = d["s11_fp64"][k]
+        e = np.abs(20 * np.log10(np.abs(s_es[sel])) - 20 * np.log10(np.abs(s_64[sel])))
+        errs.append(float(np.max(e)))
+        fall.append(logr > FALLBACK_LOG_RATIO)
+    errs, fall = np.array(errs), np.array(fall)
+    with_fb = np.where(fall, 0.0, errs)
+    return {
+        "variants": int(len(errs)),
+        "max_err_db_median": round(float(np.median(errs)), 4),
+        "max_err_db_p90": round(float(np.percentile(errs, 90)), 4),
+        "within_0p1db_no_fallback": round(float(np.mean(errs <= 0.1)), 4),
+        "fallback_rate": round(float(np.mean(fall)), 4),
+        "within_0p1db_with_fallback": round(float(np.mean(with_fb <= 0.1)), 4),
+        "max_err_db_with_fallback_p90": round(float(np.percentile(with_fb, 90)), 4),
+    }
+
+
+def time_to_solution(weights_path, first, count, seed=4, workdir="/tmp/es-tts"):
+    """Per variant, sequentially: the full-length FP64 run to S11 against the FP32 run to
+    step 1500 + network + S11 (plus the FP64 run when the fallback triggers).  Builds,
+    uploads and autotuning happen before each clock starts."""
+    from .problem import read_problem
+    from .sim import HipSim
+    from .sweep import build_one, s11
+    paths, _ = _scenarios(first, count, seed, workdir)
+    freqs = frequency_grid({"maxFreq": 6e9, "step": 6e6})
+    net = Predictor(weights_path)
+    rows = []
+    for path in paths:
+        p64 = read_problem(build_one(path, workdir, fp32=False))
+        p32 = read_problem(build_one(path, workdir, fp32=True))
+        dt = p64.grid["dt"]
+        _tail_phases(dt, freqs)                       # built once per dt, outside the clock
+        sim = HipSim(p64, precision="fp64", boundary="matlab", port_dft_freqs=freqs, farfield=False)
+        sim.autotune(3)
+        sim.synchronize()
+        t0 = time.perf_counter()
+        sim.run_no_readback(0, STEPS)
+        S64 = s11(sim, p64)
+        t64 = time.perf_counter() - t0
+        sim.close()
+        sim = HipSim(p32, precision="fp32", boundary="matlab", port_dft_freqs=freqs, farfield=False)
+        sim.autotune(3)
+        sim.synchronize()
+        t0 = time.perf_counter()
+        v, i = sim.run(0, N_PREFIX)
+        X = sim.port_dft()
+        x, scale = features(v[0], i[0])
+        tail, logr = decode(net(x)[0], x)
+        Vt, It = tail_dft(tail, scale, dt, freqs)
+        S = s11_from(X[0] + Vt, X[sim.n_sv] + It)
+        tes = time.perf_counter() - t0
+        sim.close()
+        fb = logr > FALLBACK_LOG_RATIO
+        err = float(np.max(np.abs(20 * np.log10(np.abs(S)) - 20 * np.log10(np.abs(S64)))))
+        rows.append({"t_fp64_s": t64, "t_early_s": tes + (t64 if fb else 0.0), "fallback": bool(fb),
+                     "max_err_db": 0.0 if fb else err})
+        for f in (p64.path, p32.path):
+            os.remove(f)
+    net.close()
+    t64 = sum(r["t_fp64_s"] for r in rows)
+    tes = sum(r["t_early_s"] for r in rows)
+    return {"variants": count, "fp64_s_mean": round(t64 / count, 4), "early_stop_s_mean": round(tes / count, 4),
+            "speedup": round(t64 / tes, 2), "fallbacks": int(sum(r["fallback"] for r in rows)),
+            "max_err_db": round(max(r["max_err_db"] for r in rows), 4)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("what", choices=["gen-train", "gen-heldout", "evaluate", "tts"])
+    ap.add_argument("--weights", default=None)
+    ap.add_argument("--heldout", default=None)
+    ap.add_argument("--first", type=int, default=0)
+    ap.add_argument("--count", type=int, default=100)
+    ap.add_argument("--seed", type=int, default=4)
+    ap.add_argument("--inflight", type=int, default=8)
+    ap.add_argument("--out", default=None)
+    a = ap.parse_args()
+    if a.what == "evaluate":
+        out = evaluate(a.weights, a.heldout)
+    elif a.what == "tts":
+        out = time_to_solution(a.weights, a.first, a.count, seed=a.seed)
+    else:
+        fn = gen_train if a.what == "gen-train" else gen_heldout
+        out = fn(a.first, a.count, a.out, seed=a.seed, inflight=a.inflight)
+    text = json.dumps(out)
+    if a.out and a.what in ("evaluate", "tts"):
+        with open(a.out, "w") as f:
+            f.write(text)
+    print(text, flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -48,11 +48,13 @@ EXPORTS = [
     "fdtd_hip_last_error", "fdtd_hip_device_count", "fdtd_hip_init", "fdtd_hip_upload_coeff",
     "fdtd_hip_upload_coeff_f32", "fdtd_hip_upload_cpml", "fdtd_hip_upload_vsource",
     "fdtd_hip_upload_svoltage", "fdtd_hip_upload_scurrent", "fdtd_hip_upload_farfield",
-    "fdtd_hip_upload_port_dft", "fdtd_hip_run_batch", "fdtd_hip_read_field", "fdtd_hip_write_field",
+    "fdtd_hip_upload_port_dft", "fdtd_hip_run_batch", "fdtd_hip_run_batch_async", "fdtd_hip_read_traces",
+    "fdtd_hip_query", "fdtd_hip_read_field", "fdtd_hip_write_field",
     "fdtd_hip_read_psi", "fdtd_hip_write_psi", "fdtd_hip_read_farfield", "fdtd_hip_read_port_dft", "fdtd_hip_radiated_power", "fdtd_hip_directivity",
     "fdtd_hip_set_ntff_mode", "fdtd_hip_get_ntff_mode", "fdtd_hip_set_graphs",
     "fdtd_hip_set_tuning", "fdtd_hip_get_tuning", "fdtd_hip_autotune",
     "fdtd_hip_get_coefficient_storage", "fdtd_hip_set_timing", "fdtd_hip_get_kernel_times", "fdtd_hip_synchronize", "fdtd_hip_destroy",
+    "fdtd_hip_es_create", "fdtd_hip_es_predict", "fdtd_hip_es_destroy",
 ]
 
 
@@ -95,6 +97,9 @@ def lib():
         L.fdtd_hip_upload_farfield.argtypes = [P, I, IP, D, I]
         L.fdtd_hip_upload_port_dft.argtypes = [P, I, D]
         L.fdtd_hip_run_batch.argtypes = [P, I, I, D, D]
+        L.fdtd_hip_run_batch_async.argtypes = [P, I, I]
+        L.fdtd_hip_read_traces.argtypes = [P, I, I, D, D]
+        L.fdtd_hip_query.argtypes = [P, IP]
         L.fdtd_hip_read_field.argtypes = [P, I, D, LL]
         L.fdtd_hip_write_field.argtypes = [P, I, D, LL]
         L.fdtd_hip_read_psi.argtypes = [P, I, I, D, LL]
@@ -115,6 +120,10 @@ def lib():
         L.fdtd_hip_synchronize.argtypes = [P]
         L.fdtd_hip_destroy.argtypes = [P]
         L.fdtd_hip_destroy.restype = None
+        L.fdtd_hip_es_create.argtypes = [C.POINTER(C.c_uint16), LL, FP, LL, C.POINTER(P)]
+        L.fdtd_hip_es_predict.argtypes = [P, I, FP, FP]
+        L.fdtd_hip_es_destroy.argtypes = [P]
+        L.fdtd_hip_es_destroy.restype = None
         _lib = L
     return _lib
 

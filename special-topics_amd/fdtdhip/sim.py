@@ -124,6 +124,22 @@ class HipSim:
         native.check(self.L.fdtd_hip_run_batch(self.h, ts0, count, _d(v), _d(i)), "run_batch")
         return v[: self.n_sv], i[: self.n_si]
 
+    def run_async(self, ts0: int, count: int):
+        """Enqueue steps [ts0, ts0+count) and return at once (see traces / idle)."""
+        native.check(self.L.fdtd_hip_run_batch_async(self.h, ts0, count), "run_batch_async")
+
+    def traces(self, ts0: int, count: int):
+        """Wait for the context's work, then (V[n_sv, count], I[n_si, count]) of [ts0, ts0+count)."""
+        v = np.zeros((max(self.n_sv, 1), count))
+        i = np.zeros((max(self.n_si, 1), count))
+        native.check(self.L.fdtd_hip_read_traces(self.h, ts0, count, _d(v), _d(i)), "read_traces")
+        return v[: self.n_sv], i[: self.n_si]
+
+    def idle(self) -> bool:
+        b = C.c_int(0)
+        native.check(self.L.fdtd_hip_query(self.h, C.byref(b)), "query")
+        return bool(b.value)
+
     def run_no_readback(self, ts0: int, count: int):
         native.check(self.L.fdtd_hip_run_batch(self.h, ts0, count, None, None), "run_batch")
 
