@@ -161,9 +161,10 @@ def ifa_study(workdir, scenario="ifa-config1-7000"):
     return out
 
 
-def sweep_study(workdir, count=64, inflight=8, seed=4):
-    """Config 4: `count` seeded IFA variants (uneven grids), FP32, every context's steps
-    replayed from captured graphs, `inflight` contexts on the GPU at once."""
+def sweep_study(workdir, count=64, batch=64, seed=4):
+    """Config 4: `count` seeded IFA variants (uneven grids), FP32, `batch` of them stepped
+    together by one launch per phase (tuned by the batch autotuner), 64-step chunks
+    replayed from a captured graph."""
     sys.path.insert(0, os.path.join(REPO, "special-topics_amd", "scenarios"))
     import gen_ifa_variants
     from fdtdhip.sweep import Sweep, build_all
@@ -179,7 +180,8 @@ def sweep_study(workdir, count=64, inflight=8, seed=4):
     t0 = time.perf_counter()
     files = build_all(paths, wd, fp32=True, jobs=max(1, min(host_cores(), 16)))
     build_s = time.perf_counter() - t0
-    res, secs, cells = Sweep("fp32", "matlab", inflight, tuning=(1, 4, 0), keep_traces=False).run(files)
+    sweep = Sweep("fp32", "matlab", keep_traces=False, batch=batch)
+    res, secs, cells = sweep.run(files)
     dims = np.array([r["dims"] for r in res])
     res_ghz = np.array([r["resonance_hz"] for r in res]) / 1e9
     smin = np.array([r["s11_min_db"] for r in res])
@@ -188,12 +190,33 @@ def sweep_study(workdir, count=64, inflight=8, seed=4):
         "workload": f"{count} seeded IFA variants (gen_ifa_variants.py seed {seed}), grids "
                     f"{dims[:, 0].min()}x{dims[:, 1].min()}-{dims[:, 1].max()}x{dims[:, 2].min()}-{dims[:, 2].max()}, "
                     "FP32, 7000 steps, matlab boundary, S11 at 1000 f, NTFF at 16 f (matrix-core blocked)",
-        "inflight": inflight, "build_s": round(build_s, 2), "seconds": round(secs, 3),
+        "batch": batch, "batch_tuning": list(sweep.batch_tuning), "build_s": round(build_s, 2),
+        "seconds": round(secs, 3),
         "mcell_updates_per_s": round(cells / secs / 1e6, 1), "ms_per_variant": round(secs / count * 1e3, 2),
         "resonance_ghz_min_median_max": [round(float(x), 3) for x in (res_ghz.min(), np.median(res_ghz), res_ghz.max())],
         "s11_min_db_median": round(float(np.median(smin)), 2),
         "peak_directivity_dbi_median": round(float(np.median(dpk)), 2),
     }
+
+
+EARLYSTOP_WEIGHTS = os.path.join(REPO, "special-topics_amd", "fdtdhip", "earlystop_weights.npz")
+
+
+def early_stop_study(count=4, batch_count=64, first=4600):
+    """Config 5 time-to-solution: the full-length FP64 run to S11 against the FP32 run
+    stopped at step 1500 + the network (HIP kernels) + S11, with the FP64 fallback when
+    the network's residual estimate exceeds 1e-4; once with `batch_count` variants in
+    one batched launch on each side, once for `count` variants one at a time."""
+    from fdtdhip import earlystop
+    if not os.path.exists(EARLYSTOP_WEIGHTS):
+        return {"skipped": f"no trained weights at {EARLYSTOP_WEIGHTS}"}
+    out = {"workload": "IFA variants (gen_ifa_variants.py seed 4), 7000 steps, matlab boundary, S11 at 1000 f; "
+                       "early stop at step 1500; variants from 4600 on are outside the training and held-out ranges"}
+    progress(f"early stop: {batch_count} variants batched on each side")
+    out["batched"] = earlystop.time_to_solution_batched(EARLYSTOP_WEIGHTS, first, batch_count)
+    progress(f"early stop: {count} variants one at a time")
+    out["single"] = earlystop.time_to_solution(EARLYSTOP_WEIGHTS, first, count)
+    return out
 
 
 def progress(msg: str):
@@ -219,6 +242,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-ifa", action="store_true", help="skip the IFA FP32-vs-FP64 time-to-solution study")
     ap.add_argument("--no-sweep", action="store_true", help="skip the 64-variant IFA sweep")
+    ap.add_argument("--no-early-stop", action="store_true", help="skip the early-stop time-to-solution study")
     ap.add_argument("--ntff-mode", default="auto", choices=["auto", "direct", "blocked"],
                     help="NTFF running DFT: step-by-step (direct) or K-blocked on the matrix cores")
     ap.add_argument("--workdir", default=os.path.join(tempfile.gettempdir(), "fdtd-bench"))
@@ -227,6 +251,8 @@ def main():
     ap.add_argument("--planes", type=int, default=-1, help="x planes per workgroup (-1 = autotune)")
     ap.add_argument("--cells", type=int, default=0, help="cells per thread (with --planes)")
     ap.add_argument("--coef", type=int, default=0, help="coefficient storage (with --planes)")
+    ap.add_argument("--fuse-cpml", type=int, default=0,
+                    help="CPML placement with --planes: 0 separate pass, 1 z slabs fused, 2 all faces fused")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -262,7 +288,7 @@ def main():
     if args.planes < 0:
         tuning = sim.autotune(3)
     else:
-        sim.set_tuning(args.planes, args.cells, args.coef)
+        sim.set_tuning(args.planes, args.cells, args.coef, args.fuse_cpml)
         tuning = sim.tuning()
     setup_s = time.perf_counter() - t_setup
 
@@ -300,7 +326,8 @@ def main():
     achieved = dom_bytes / (avg_ms[dom] * 1e-3) / 1e9
 
     ctype = "float" if fp32 else "double"
-    kname = f"fdtd::k_update_{'h' if dom == 0 else 'e'}<{ctype}, {coef_mode}, {max(tuning[1], 1)}>"
+    kname = (f"fdtd::k_update_{'h' if dom == 0 else 'e'}<{ctype}, {coef_mode}, {max(tuning[1], 1)}, "
+             f"{'true' if tuning[3] else 'false'}>")
     traffic = measured_traffic(args.workload, args.precision, kname)
 
     mcells = ncells * args.steps / elapsed / 1e6
@@ -338,7 +365,8 @@ def main():
         "hbm_equivalent_frac_step": round((bytes_h + bytes_e) / (elapsed / args.steps) / 1e9 / HBM_PEAK_GBS, 4),
         "setup_s": round(setup_s, 2),
         "tuning": {"planes_per_workgroup": tuning[0], "cells_per_thread": tuning[1],
-                   "coefficient_storage": ["indexed", "arrays"][tuning[2]]},
+                   "coefficient_storage": ["indexed", "arrays"][tuning[2]],
+                   "cpml": ["separate pass", "z slabs fused into bulk", "all faces fused into bulk"][tuning[3]]},
     }
 
     if not args.no_ifa:
@@ -348,6 +376,10 @@ def main():
     if not args.no_sweep:
         progress("64-variant IFA sweep")
         line["sweep"] = sweep_study(args.workdir)
+
+    if not args.no_early_stop:
+        progress("early-stop time-to-solution")
+        line["early_stop"] = early_stop_study()
 
     if not args.no_cpu_baseline:
         progress("CPU baseline")

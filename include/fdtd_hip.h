@@ -202,6 +202,10 @@ typedef struct {
     int coefficient_storage;   /* 0: deduplicated (self, curl, curl) triples with 8- or
                                   16-bit per-cell indices when they fit, else per-cell
                                   arrays; 1: per-cell arrays */
+    int fuse_cpml;             /* 0: all CPML terms in the separate CPML pass; 1: cells in
+                                  the z-face slabs get all their CPML terms inside the
+                                  bulk kernels, right after their bulk update; 2: every
+                                  CPML cell does, and there is no separate pass */
 } FdtdHipTuning;
 
 int fdtd_hip_set_tuning(FdtdHipContext *ctx, const FdtdHipTuning *tuning);
@@ -226,6 +230,35 @@ int fdtd_hip_synchronize(FdtdHipContext *ctx);
 
 /* Free all device state.  Replaces fdtd_destroy (fdtd_cuda.h:63). */
 void fdtd_hip_destroy(FdtdHipContext *ctx);
+
+/* Many problems ("variants", of any grid size) stepped together: one launch per
+ * phase covers every variant (the H and E stencils with all CPML faces fused, the port
+ * epilogue, the NTFF surface work), and whole 64-step chunks replay a captured HIP
+ * graph.  Each variant's results are bit-identical to running its context alone with
+ * the same tuning, and are read with the per-context calls above (read_traces,
+ * read_port_dft, read_farfield, ...) once the run has finished.  No counterpart in
+ * the reference, which runs one problem per process (the MATLAB sweep scripts loop
+ * over scenarios).
+ *
+ * The contexts are borrowed, not owned: they must share a device and precision and
+ * must not be run on their own while a batched run is in flight.  Tuning: planes per
+ * workgroup (0 = each variant's default), cells per thread and coefficient storage as
+ * in FdtdHipTuning; fuse_cpml must be 0 or 2 (every face is fused).  When the
+ * variants' deduplicated coefficient tables differ in index width, per-cell arrays are
+ * used for all. */
+typedef struct FdtdHipMulti FdtdHipMulti;
+int fdtd_hip_multi_create(FdtdHipContext *const *ctxs, int n, const FdtdHipTuning *tuning, FdtdHipMulti **out);
+/* Steps [tsStart, tsStart+count) of every variant, enqueued; each context's own stream
+ * waits for them, so per-context reads afterwards see the result. */
+int fdtd_hip_multi_run_async(FdtdHipMulti *m, int tsStart, int count);
+int fdtd_hip_multi_synchronize(FdtdHipMulti *m);
+/* As fdtd_hip_set_graphs, for the batch (default on). */
+int fdtd_hip_multi_set_graphs(FdtdHipMulti *m, int enable);
+int fdtd_hip_multi_get_tuning(FdtdHipMulti *m, FdtdHipTuning *tuning);
+/* Time the batched launches for every cells-per-thread / planes / storage choice over a
+ * few steps, keep the fastest, and reset every variant (before its first step only). */
+int fdtd_hip_multi_autotune(FdtdHipMulti *m, int steps_per_candidate, FdtdHipTuning *chosen);
+void fdtd_hip_multi_destroy(FdtdHipMulti *m);
 
 /* Early-stop network (no counterpart in the reference): predicts the port-voltage and
  * port-current tail of an IFA run from its first 1500 steps (architecture and data

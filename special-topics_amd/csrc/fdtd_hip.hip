@@ -7,6 +7,7 @@
 #include "fdtd_kernels.hpp"
 #include "ntff_post.hpp"
 #include "ntff_block.hpp"
+#include "multi_kernels.hpp"
 #include "earlystop_net.hpp"
 
 #include <hip/hip_runtime.h>
@@ -104,9 +105,9 @@ struct FdtdHipContext {
   int use_graphs = 1;
   int *d_step = nullptr;
   hipGraphExec_t graph = nullptr;
-  int graph_key[6] = {-1, -1, -1, -1, -1, -1};
+  int graph_key[7] = {-1, -1, -1, -1, -1, -1, -1};
 
-  FdtdHipTuning tune{0, 0, 0};  // requested launch shape / coefficient storage (0 = default)
+  FdtdHipTuning tune{0, 0, 0, 0};  // requested launch shape / coefficient storage (0 = default)
   int coef_mode = -1;           // CoefMode in use; -1 = not decided yet
   int coef_built = 0;           // deduplicated tables built (idx width in coef_idx_bytes)
   int coef_idx_bytes = 0;       // 0 = triples do not fit 16 bits
@@ -156,7 +157,7 @@ void drop_graph(FdtdHipContext *c) {
 
 template <typename T>
 Grid<T> make_grid(const FdtdHipContext *c) {
-  Grid<T> g;
+  Grid<T> g{};
   g.nx = c->d.nx; g.ny = c->d.ny; g.nz = c->d.nz;
   g.SJ = c->SJ; g.SI = c->SI;
   g.Hx = (T *)c->field_ptr(0); g.Hy = (T *)c->field_ptr(1); g.Hz = (T *)c->field_ptr(2);
@@ -173,7 +174,7 @@ Grid<T> make_grid(const FdtdHipContext *c) {
 
 template <typename T>
 FaceSet<T> make_faces(const FdtdHipContext *c) {
-  FaceSet<T> fs;
+  FaceSet<T> fs{};
   for (int f = 0; f < 6; ++f) {
     const auto &h = c->face[f];
     Face<T> &F = fs.f[f];
@@ -372,41 +373,64 @@ MarchGeom march_geom(const FdtdHipContext *c, int planes) {
   return m;
 }
 
-template <typename T, int CM>
-void launch_h_w(FdtdHipContext *c, const Grid<T> &g, const MarchGeom &m) {
+int fuse_cpml(const FdtdHipContext *c) { return c->tune.fuse_cpml; }
+
+template <typename T, int CM, int FC>
+void launch_h_w(FdtdHipContext *c, const Grid<T> &g, const MarchGeom &m, const FaceSet<T> &fs) {
   const dim3 grid(8 * m.per_xcd), block(256);
   switch (cells_per_thread(c)) {
-    case 4: hipLaunchKernelGGL((k_update_h<T, CM, 4>), grid, block, 0, c->stream, g, m); break;
-    case 2: hipLaunchKernelGGL((k_update_h<T, CM, 2>), grid, block, 0, c->stream, g, m); break;
-    default: hipLaunchKernelGGL((k_update_h<T, CM, 1>), grid, block, 0, c->stream, g, m); break;
+    case 4: hipLaunchKernelGGL((k_update_h<T, CM, 4, FC>), grid, block, 0, c->stream, g, m, fs); break;
+    case 2: hipLaunchKernelGGL((k_update_h<T, CM, 2, FC>), grid, block, 0, c->stream, g, m, fs); break;
+    default: hipLaunchKernelGGL((k_update_h<T, CM, 1, FC>), grid, block, 0, c->stream, g, m, fs); break;
   }
 }
 
-template <typename T, int CM>
-void launch_e_w(FdtdHipContext *c, const Grid<T> &g, const MarchGeom &m, int ghost) {
+template <typename T, int CM, int FC>
+void launch_e_w(FdtdHipContext *c, const Grid<T> &g, const MarchGeom &m, int ghost, const FaceSet<T> &fs) {
   const dim3 grid(8 * m.per_xcd), block(256);
   switch (cells_per_thread(c)) {
-    case 4: hipLaunchKernelGGL((k_update_e<T, CM, 4>), grid, block, 0, c->stream, g, m, ghost); break;
-    case 2: hipLaunchKernelGGL((k_update_e<T, CM, 2>), grid, block, 0, c->stream, g, m, ghost); break;
-    default: hipLaunchKernelGGL((k_update_e<T, CM, 1>), grid, block, 0, c->stream, g, m, ghost); break;
+    case 4: hipLaunchKernelGGL((k_update_e<T, CM, 4, FC>), grid, block, 0, c->stream, g, m, ghost, fs); break;
+    case 2: hipLaunchKernelGGL((k_update_e<T, CM, 2, FC>), grid, block, 0, c->stream, g, m, ghost, fs); break;
+    default: hipLaunchKernelGGL((k_update_e<T, CM, 1, FC>), grid, block, 0, c->stream, g, m, ghost, fs); break;
+  }
+}
+
+template <typename T, int FC>
+void launch_h_f(FdtdHipContext *c, const Grid<T> &g, const MarchGeom &m, const FaceSet<T> &fs) {
+  if (c->coef_mode == COEF_U8) launch_h_w<T, COEF_U8, FC>(c, g, m, fs);
+  else if (c->coef_mode == COEF_U16) launch_h_w<T, COEF_U16, FC>(c, g, m, fs);
+  else launch_h_w<T, COEF_ARRAYS, FC>(c, g, m, fs);
+}
+
+template <typename T, int FC>
+void launch_e_f(FdtdHipContext *c, const Grid<T> &g, const MarchGeom &m, int ghost, const FaceSet<T> &fs) {
+  if (c->coef_mode == COEF_U8) launch_e_w<T, COEF_U8, FC>(c, g, m, ghost, fs);
+  else if (c->coef_mode == COEF_U16) launch_e_w<T, COEF_U16, FC>(c, g, m, ghost, fs);
+  else launch_e_w<T, COEF_ARRAYS, FC>(c, g, m, ghost, fs);
+}
+
+template <typename T>
+void launch_h(FdtdHipContext *c, const Grid<T> &g, const MarchGeom &m, const FaceSet<T> &fs) {
+  switch (fuse_cpml(c)) {
+    case CPML_FUSE_ALL: launch_h_f<T, CPML_FUSE_ALL>(c, g, m, fs); break;
+    case CPML_FUSE_Z: launch_h_f<T, CPML_FUSE_Z>(c, g, m, fs); break;
+    default: launch_h_f<T, CPML_SEPARATE>(c, g, m, fs); break;
   }
 }
 
 template <typename T>
-void launch_h(FdtdHipContext *c, const Grid<T> &g, const MarchGeom &m) {
-  if (c->coef_mode == COEF_U8) launch_h_w<T, COEF_U8>(c, g, m);
-  else if (c->coef_mode == COEF_U16) launch_h_w<T, COEF_U16>(c, g, m);
-  else launch_h_w<T, COEF_ARRAYS>(c, g, m);
+void launch_e(FdtdHipContext *c, const Grid<T> &g, const MarchGeom &m, int ghost, const FaceSet<T> &fs) {
+  switch (fuse_cpml(c)) {
+    case CPML_FUSE_ALL: launch_e_f<T, CPML_FUSE_ALL>(c, g, m, ghost, fs); break;
+    case CPML_FUSE_Z: launch_e_f<T, CPML_FUSE_Z>(c, g, m, ghost, fs); break;
+    default: launch_e_f<T, CPML_SEPARATE>(c, g, m, ghost, fs); break;
+  }
 }
 
-template <typename T>
-void launch_e(FdtdHipContext *c, const Grid<T> &g, const MarchGeom &m, int ghost) {
-  if (c->coef_mode == COEF_U8) launch_e_w<T, COEF_U8>(c, g, m, ghost);
-  else if (c->coef_mode == COEF_U16) launch_e_w<T, COEF_U16>(c, g, m, ghost);
-  else launch_e_w<T, COEF_ARRAYS>(c, g, m, ghost);
-}
-
-// Disjoint boxes covering every cell a CPML face of the phase (0 = H, 1 = E) acts on.
+// Disjoint boxes covering every cell a CPML face of the phase (0 = H, 1 = E) acts on
+// that the bulk kernels leave to k_cpml: all of them with CPML_SEPARATE; with
+// CPML_FUSE_Z all but the z-slab cells of the planes the bulk kernels cover (H: all;
+// E: i < nx); none with CPML_FUSE_ALL.
 CpmlBoxes cpml_boxes(const FdtdHipContext *c, int phase) {
   const int n[3] = {c->d.nx, c->d.ny, c->d.nz};
   std::vector<std::pair<int, int>> in[3], out[3];
@@ -434,9 +458,21 @@ CpmlBoxes cpml_boxes(const FdtdHipContext *c, int phase) {
     bx.start[b + 1] = bx.start[b] + (long long)bx.ext[b][0] * bx.ext[b][1] * bx.ext[b][2];
   };
   const std::pair<int, int> fy{0, n[1] + 1}, fz{0, n[2] + 1};
-  for (auto &x : in[0]) add(x, fy, fz);
-  for (auto &x : out[0]) for (auto &y : in[1]) add(x, y, fz);
-  for (auto &x : out[0]) for (auto &y : out[1]) for (auto &z : in[2]) add(x, y, z);
+  if (fuse_cpml(c) == CPML_FUSE_ALL) return bx;
+  if (fuse_cpml(c) == CPML_SEPARATE) {
+    for (auto &x : in[0]) add(x, fy, fz);
+    for (auto &x : out[0]) for (auto &y : in[1]) add(x, y, fz);
+    for (auto &x : out[0]) for (auto &y : out[1]) for (auto &z : in[2]) add(x, y, z);
+    return bx;
+  }
+  for (auto &x : in[0]) for (auto &z : out[2]) add(x, fy, z);
+  for (auto &x : out[0]) for (auto &y : in[1]) for (auto &z : out[2]) add(x, y, z);
+  const int i_end = phase == 0 ? n[0] + 1 : n[0];          // first plane the bulk does not cover
+  for (int a = 0; a < 2; ++a)
+    for (auto &x : a == 0 ? in[0] : out[0]) {
+      const std::pair<int, int> xr{std::max(x.first, i_end), x.second};
+      for (auto &z : in[2]) add(xr, fy, z);
+    }
   return bx;
 }
 
@@ -465,7 +501,7 @@ int ensure_ntff_buffer(FdtdHipContext *c) {
 
 template <typename T>
 NtffBlock<T> ntff_block(const FdtdHipContext *c) {
-  NtffBlock<T> nb;
+  NtffBlock<T> nb{};
   nb.buf = (T *)c->ntff_buf;
   nb.spad = c->ntff_spad;
   for (int a = 0; a < 25; ++a) nb.base[a] = c->ntff_base[a];
@@ -510,13 +546,13 @@ int launch_step(FdtdHipContext *c, int ts, const int *tsp = nullptr) {
   const bool t = c->timing;
 
   if (t) HIP_TRY(hipEventRecord(c->ev[0], c->stream));
-  launch_h<T>(c, g, mh);
+  launch_h<T>(c, g, mh, fs);
   if (t) HIP_TRY(hipEventRecord(c->ev[1], c->stream));
   if (c->cpml_h.nbox)
     hipLaunchKernelGGL(k_cpml_h<T>, dim3((unsigned)((c->cpml_h.start[c->cpml_h.nbox] + 255) / 256)), block, 0,
                        c->stream, g, fs, c->cpml_h);
   if (t) HIP_TRY(hipEventRecord(c->ev[2], c->stream));
-  launch_e<T>(c, g, me, ghost);
+  launch_e<T>(c, g, me, ghost, fs);
   if (t) HIP_TRY(hipEventRecord(c->ev[3], c->stream));
   if (c->cpml_e.nbox)
     hipLaunchKernelGGL(k_cpml_e<T>, dim3((unsigned)((c->cpml_e.start[c->cpml_e.nbox] + 255) / 256)), block, 0,
@@ -568,10 +604,10 @@ int launch_step(FdtdHipContext *c, int ts, const int *tsp = nullptr) {
 constexpr int GRAPH_STEPS = NTFF_K;
 
 // Everything a captured step depends on besides the device data itself.
-void graph_key(const FdtdHipContext *c, int key[6]) {
+void graph_key(const FdtdHipContext *c, int key[7]) {
   key[0] = c->tune.planes_per_workgroup; key[1] = c->tune.cells_per_thread;
   key[2] = c->tune.coefficient_storage; key[3] = c->coef_mode;
-  key[4] = ntff_blocked(c) ? 1 : 0; key[5] = c->ndft;
+  key[4] = ntff_blocked(c) ? 1 : 0; key[5] = c->ndft; key[6] = c->tune.fuse_cpml;
 }
 
 // Capture GRAPH_STEPS steps plus the step advance into an executable graph.
@@ -602,7 +638,7 @@ int run_steps(FdtdHipContext *c, int ts, int count) {
     const int t = ts + s;
     const bool aligned = !ntff_blocked(c) || (c->blk_rows == 0 && c->blk_t0 == t);
     if (c->use_graphs && !c->timing && count - s >= GRAPH_STEPS && aligned) {
-      int key[6];
+      int key[7];
       graph_key(c, key);
       if (!c->graph || std::memcmp(key, c->graph_key, sizeof key) != 0) {
         if (build_graph<T>(c)) return FDTD_HIP_ERR;
@@ -1109,6 +1145,8 @@ static int check_tuning(const FdtdHipContext *c, const FdtdHipTuning *t, const c
     return fail("%s: %d cells per thread unsupported (1, 2%s)", who, t->cells_per_thread, maxw == 4 ? " or 4" : "");
   if (t->coefficient_storage < 0 || t->coefficient_storage > 1)
     return fail("%s: coefficient storage %d unknown", who, t->coefficient_storage);
+  if (t->fuse_cpml < CPML_SEPARATE || t->fuse_cpml > CPML_FUSE_ALL)
+    return fail("%s: fuse_cpml %d unknown (0, 1 or 2)", who, t->fuse_cpml);
   return FDTD_HIP_OK;
 }
 
@@ -1168,13 +1206,16 @@ int fdtd_hip_autotune(FdtdHipContext *c, int steps_per_candidate, FdtdHipTuning 
     if (cs == 0 && !c->coef_idx_bytes) continue;
     for (int w : ws) {
       if (w == 4 && !c->fp32()) continue;
-      for (int pl : planes) cands.push_back(FdtdHipTuning{pl, w, cs});
+      for (int pl : planes)
+        for (int fc = CPML_SEPARATE; fc <= CPML_FUSE_ALL; ++fc) cands.push_back(FdtdHipTuning{pl, w, cs, fc});
     }
   }
   double best = 1e300;
   FdtdHipTuning best_t = saved;
   for (const auto &t : cands) {
     c->tune = t;
+    c->cpml_h = cpml_boxes(c, 0);
+    c->cpml_e = cpml_boxes(c, 1);
     c->coef_mode = -1;
     if (finalize_coefficients(c)) return FDTD_HIP_ERR;
     c->timing = 1;
@@ -1243,6 +1284,433 @@ void fdtd_hip_destroy(FdtdHipContext *c) {
   delete c;
 }
 
+
+// ------------------------------------------------------------- many problems --
+
+struct FdtdHipMulti {
+  std::vector<FdtdHipContext *> ctx;
+  int device = 0;
+  bool fp32 = false;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev = nullptr;
+  FdtdHipTuning tune{0, 0, 0, CPML_FUSE_ALL};
+  int cm = -1;                  // coefficient storage shared by every variant
+  int ntff = 0;                 // 0: no variant has NTFF, 1: direct, 2: blocked
+  bool ports = false;
+  // Device descriptors and workgroup maps, with the host images last uploaded.
+  void *d_desc = nullptr;
+  unsigned *d_map = nullptr;    // [H tiles | E tiles | NTFF blocks]
+  size_t desc_cap = 0, map_cap = 0;
+  std::vector<char> h_desc;
+  std::vector<unsigned> h_map;
+  int nh = 0, ne = 0, nfb = 0;  // H tiles, E tiles, NTFF blocks
+  int *d_step = nullptr;
+  hipGraphExec_t graph = nullptr;
+  int use_graphs = 1;
+  std::vector<hipStream_t> saved;   // the contexts' own streams while they are lent
+};
+
+extern "C++" {
+namespace {
+
+void multi_drop_graph(FdtdHipMulti *m) {
+  if (m->graph) (void)hipGraphExecDestroy(m->graph);
+  m->graph = nullptr;
+}
+
+// Lend every context's stream to the batch: the batch stream first waits for the
+// work already queued on each, and the contexts then launch on the batch stream.
+int multi_borrow(FdtdHipMulti *m) {
+  m->saved.resize(m->ctx.size());
+  for (size_t v = 0; v < m->ctx.size(); ++v) {
+    FdtdHipContext *c = m->ctx[v];
+    HIP_TRY(hipEventRecord(m->ev, c->stream));
+    HIP_TRY(hipStreamWaitEvent(m->stream, m->ev, 0));
+    m->saved[v] = c->stream;
+    c->stream = m->stream;
+  }
+  return FDTD_HIP_OK;
+}
+
+// Give the streams back; each waits for everything the batch queued so far.
+int multi_return(FdtdHipMulti *m) {
+  if (m->saved.size() != m->ctx.size()) return FDTD_HIP_OK;
+  HIP_TRY(hipEventRecord(m->ev, m->stream));
+  for (size_t v = 0; v < m->ctx.size(); ++v) {
+    m->ctx[v]->stream = m->saved[v];
+    HIP_TRY(hipStreamWaitEvent(m->saved[v], m->ev, 0));
+  }
+  m->saved.clear();
+  return FDTD_HIP_OK;
+}
+
+template <typename T>
+void put_map(std::vector<unsigned> &map, int v, long n) {
+  for (long q = 0; q < n; ++q) map.push_back(((unsigned)v << MULTI_LOCAL_BITS) | (unsigned)q);
+}
+
+// Apply the batch tuning to every context, make the coefficient storage uniform,
+// and (re)upload the descriptors and maps when they changed.
+template <typename T>
+int multi_prepare(FdtdHipMulti *m) {
+  for (int pass = 0; pass < 2; ++pass) {
+    for (FdtdHipContext *c : m->ctx) {
+      FdtdHipTuning t = m->tune;
+      t.fuse_cpml = CPML_FUSE_ALL;
+      if (pass == 1) t.coefficient_storage = 1;
+      if (std::memcmp(&t, &c->tune, sizeof t) != 0) {
+        c->tune = t;
+        c->coef_mode = -1;
+        drop_graph(c);
+      }
+      if (ensure_traces(c) || finalize_coefficients(c)) return FDTD_HIP_ERR;
+      c->cpml_h = cpml_boxes(c, 0);
+      c->cpml_e = cpml_boxes(c, 1);
+      if (ntff_blocked(c) && ensure_ntff_buffer(c)) return FDTD_HIP_ERR;
+    }
+    bool same = true;
+    for (FdtdHipContext *c : m->ctx) same = same && c->coef_mode == m->ctx[0]->coef_mode;
+    if (same) break;
+    if (pass == 1) return fail("multi: coefficient storage differs between variants");
+  }
+  m->cm = m->ctx[0]->coef_mode;
+  m->ntff = 0;
+  m->ports = false;
+  for (FdtdHipContext *c : m->ctx) {
+    if (c->nff) {
+      const int mode = ntff_blocked(c) ? 2 : 1;
+      if (m->ntff && m->ntff != mode) return fail("multi: variants mix blocked and direct NTFF accumulation");
+      m->ntff = mode;
+    }
+    if (c->ports.nvs + c->ports.nsv + c->ports.nsi > 0) m->ports = true;
+  }
+
+  const int n = (int)m->ctx.size();
+  std::vector<VariantDesc<T>> desc(n);
+  std::vector<unsigned> hmap, emap, fmap;
+  for (int v = 0; v < n; ++v) {
+    FdtdHipContext *c = m->ctx[v];
+    VariantDesc<T> &d = desc[v];
+    std::memset(&d, 0, sizeof d);
+    d.g = make_grid<T>(c);
+    d.fs = make_faces<T>(c);
+    d.ghost = c->d.boundary == FDTD_HIP_BOUNDARY_REFERENCE;
+    d.mh = march_geom(c, c->d.nx + 1);
+    d.me = march_geom(c, d.ghost ? c->d.nx : c->d.nx + 1);
+    d.ps = c->ports;
+    d.ps.ndft = c->ndft; d.ps.w = c->dft_w; d.ps.dft = c->dft_acc;
+    d.has_ports = c->ports.nvs + c->ports.nsv + c->ports.nsi > 0;
+    d.ff = c->ff;
+    if (ntff_blocked(c)) d.nb = ntff_block<T>(c);
+    const long th = (long)d.mh.ncol * d.mh.nchunk, te = (long)d.me.ncol * d.me.nchunk;
+    long tf = 0;
+    if (c->nff) {
+      const FarfieldState &f = c->ff;
+      tf = (2 * ((long)f.nj * f.nk + (long)f.ni * f.nk + (long)f.ni * f.nj) + 255) / 256;
+    }
+    if (th > (long)MULTI_LOCAL_MASK || te > (long)MULTI_LOCAL_MASK || tf > (long)MULTI_LOCAL_MASK)
+      return fail("multi: variant %d is too large for a batched launch", v);
+    put_map<T>(hmap, v, th);
+    put_map<T>(emap, v, te);
+    put_map<T>(fmap, v, tf);
+  }
+  m->nh = (int)hmap.size();
+  m->ne = (int)emap.size();
+  m->nfb = (int)fmap.size();
+  std::vector<unsigned> map(hmap);
+  map.insert(map.end(), emap.begin(), emap.end());
+  map.insert(map.end(), fmap.begin(), fmap.end());
+  // The launch choices a captured graph bakes in besides the device images.
+  map.push_back((unsigned)m->cm);
+  map.push_back((unsigned)cells_per_thread(m->ctx[0]));
+  map.push_back((unsigned)m->ntff);
+  map.push_back(m->ports ? 1u : 0u);
+  const size_t dbytes = desc.size() * sizeof(VariantDesc<T>);
+  const bool changed = dbytes != m->h_desc.size() || std::memcmp(desc.data(), m->h_desc.data(), dbytes) != 0 ||
+                       map != m->h_map;
+  if (!changed) return FDTD_HIP_OK;
+  multi_drop_graph(m);
+  HIP_TRY(hipStreamSynchronize(m->stream));     // no queued launch still reads the old images
+  if (dbytes > m->desc_cap) {
+    (void)hipFree(m->d_desc);
+    m->d_desc = nullptr;
+    HIP_TRY(hipMalloc(&m->d_desc, dbytes));
+    m->desc_cap = dbytes;
+  }
+  if (map.size() * sizeof(unsigned) > m->map_cap || !m->d_map) {
+    (void)hipFree(m->d_map);
+    m->d_map = nullptr;
+    const size_t bytes = std::max<size_t>(map.size(), 1) * sizeof(unsigned);
+    HIP_TRY(hipMalloc((void **)&m->d_map, bytes));
+    m->map_cap = bytes;
+  }
+  m->h_desc.assign((const char *)desc.data(), (const char *)desc.data() + dbytes);
+  m->h_map = map;
+  HIP_TRY(hipMemcpyAsync(m->d_desc, m->h_desc.data(), dbytes, hipMemcpyHostToDevice, m->stream));
+  if (!map.empty())
+    HIP_TRY(hipMemcpyAsync(m->d_map, m->h_map.data(), map.size() * sizeof(unsigned), hipMemcpyHostToDevice, m->stream));
+  HIP_TRY(hipStreamSynchronize(m->stream));     // the host images may change before the next run
+  return FDTD_HIP_OK;
+}
+
+template <typename T, int CM, int W>
+void multi_launch_hw(FdtdHipMulti *m, bool e_phase) {
+  const VariantDesc<T> *vd = (const VariantDesc<T> *)m->d_desc;
+  const int nt = e_phase ? m->ne : m->nh;
+  const unsigned *map = m->d_map + (e_phase ? m->nh : 0);
+  const int per_xcd = (nt + 7) / 8;
+  if (e_phase)
+    hipLaunchKernelGGL((k_multi_update_e<T, CM, W, CPML_FUSE_ALL>), dim3(8 * per_xcd), dim3(256), 0, m->stream, vd,
+                       map, nt, per_xcd);
+  else
+    hipLaunchKernelGGL((k_multi_update_h<T, CM, W, CPML_FUSE_ALL>), dim3(8 * per_xcd), dim3(256), 0, m->stream, vd,
+                       map, nt, per_xcd);
+}
+
+template <typename T, int CM>
+void multi_launch_hc(FdtdHipMulti *m, bool e_phase) {
+  switch (cells_per_thread(m->ctx[0])) {
+    case 4: multi_launch_hw<T, CM, 4>(m, e_phase); break;
+    case 2: multi_launch_hw<T, CM, 2>(m, e_phase); break;
+    default: multi_launch_hw<T, CM, 1>(m, e_phase); break;
+  }
+}
+
+template <typename T>
+void multi_launch_phase(FdtdHipMulti *m, bool e_phase) {
+  if (m->cm == COEF_U8) multi_launch_hc<T, COEF_U8>(m, e_phase);
+  else if (m->cm == COEF_U16) multi_launch_hc<T, COEF_U16>(m, e_phase);
+  else multi_launch_hc<T, COEF_ARRAYS>(m, e_phase);
+}
+
+// One time step of every variant (see launch_step for ts / tsp).
+template <typename T>
+int multi_step(FdtdHipMulti *m, int ts, const int *tsp = nullptr) {
+  const VariantDesc<T> *vd = (const VariantDesc<T> *)m->d_desc;
+  if (m->nh) multi_launch_phase<T>(m, false);
+  if (m->ne) multi_launch_phase<T>(m, true);
+  if (m->ports)
+    hipLaunchKernelGGL(k_multi_port_epilogue<T>, dim3((unsigned)m->ctx.size()), dim3(256), 0, m->stream, vd, tsp, ts);
+  if (m->ntff == 1) {
+    hipLaunchKernelGGL(k_multi_ntff_direct<T>, dim3((unsigned)m->nfb), dim3(256), 0, m->stream, vd,
+                       m->d_map + m->nh + m->ne, tsp, ts);
+  } else if (m->ntff == 2) {
+    // Every variant's block state moves in lockstep (multi_run restarts them together).
+    FdtdHipContext *lead = nullptr;
+    for (FdtdHipContext *c : m->ctx) if (c->nff) { lead = c; break; }
+    const int row = tsp ? ts : lead->blk_rows;
+    hipLaunchKernelGGL(k_multi_ntff_gather<T>, dim3((unsigned)m->nfb), dim3(256), 0, m->stream, vd,
+                       m->d_map + m->nh + m->ne, row);
+    for (FdtdHipContext *c : m->ctx) {
+      if (!c->nff) continue;
+      if (tsp) {
+        if (ts == NTFF_K - 1) ntff_fold<T>(c, tsp, 0, 0, NTFF_K);
+      } else {
+        ++c->blk_rows;
+        if (c->blk_rows == NTFF_K && ntff_flush<T>(c)) return FDTD_HIP_ERR;
+      }
+    }
+  }
+  HIP_TRY(hipGetLastError());
+  return FDTD_HIP_OK;
+}
+
+template <typename T>
+int multi_build_graph(FdtdHipMulti *m) {
+  multi_drop_graph(m);
+  HIP_TRY(hipStreamBeginCapture(m->stream, hipStreamCaptureModeThreadLocal));
+  int rc = FDTD_HIP_OK;
+  for (int k = 0; k < GRAPH_STEPS && !rc; ++k) rc = multi_step<T>(m, k, m->d_step);
+  if (!rc) hipLaunchKernelGGL(k_advance_step, dim3(1), dim3(64), 0, m->stream, m->d_step, GRAPH_STEPS);
+  hipGraph_t g = nullptr;
+  const hipError_t end = hipStreamEndCapture(m->stream, &g);
+  if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
+  if (end != hipSuccess) return fail("multi: graph capture failed: %s", hipGetErrorString(end));
+  const hipError_t inst = hipGraphInstantiate(&m->graph, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (inst != hipSuccess) { m->graph = nullptr; return fail("multi: graph instantiation failed: %s", hipGetErrorString(inst)); }
+  return FDTD_HIP_OK;
+}
+
+// Steps [ts, ts + count) of every variant, on the batch stream (contexts' streams lent).
+template <typename T>
+int multi_steps(FdtdHipMulti *m, int ts, int count, bool graphs) {
+  if (m->ntff == 2) {
+    // Continue the open blocks when every variant's block ends right at ts with the
+    // same row count (as a single context continues its own), else fold what is left
+    // and start new blocks at ts.
+    int rows = -1;
+    bool cont = true;
+    for (FdtdHipContext *c : m->ctx) {
+      if (!c->nff) continue;
+      if (rows < 0) rows = c->blk_rows;
+      cont = cont && c->blk_rows == rows && c->blk_t0 + c->blk_rows == ts;
+    }
+    for (FdtdHipContext *c : m->ctx) {
+      if (!c->nff || cont) continue;
+      if (ntff_flush<T>(c)) return FDTD_HIP_ERR;
+      c->blk_t0 = ts;
+      c->blk_rows = c->blk_flushed = 0;
+    }
+  }
+  bool base_set = false;
+  for (int s = 0; s < count;) {
+    const int t = ts + s;
+    bool aligned = true;
+    for (FdtdHipContext *c : m->ctx)
+      if (c->nff && ntff_blocked(c)) aligned = aligned && c->blk_rows == 0 && c->blk_t0 == t;
+    if (graphs && count - s >= GRAPH_STEPS && aligned) {
+      if (!m->graph) {
+        if (multi_build_graph<T>(m)) return FDTD_HIP_ERR;
+        base_set = false;
+      }
+      if (!base_set) hipLaunchKernelGGL(k_set_step, dim3(1), dim3(64), 0, m->stream, m->d_step, t);
+      base_set = true;
+      HIP_TRY(hipGraphLaunch(m->graph, m->stream));
+      for (FdtdHipContext *c : m->ctx) if (c->nff && ntff_blocked(c)) c->blk_t0 = t + GRAPH_STEPS;
+      s += GRAPH_STEPS;
+    } else {
+      if (multi_step<T>(m, t)) return FDTD_HIP_ERR;
+      base_set = false;
+      ++s;
+    }
+  }
+  if (m->ntff == 2)
+    for (FdtdHipContext *c : m->ctx)
+      if (c->nff && ntff_flush<T>(c)) return FDTD_HIP_ERR;
+  HIP_TRY(hipGetLastError());
+  return FDTD_HIP_OK;
+}
+
+int multi_run(FdtdHipMulti *m, int ts, int count, bool graphs) {
+  for (FdtdHipContext *c : m->ctx)
+    if (ts < 0 || count < 0 || ts + count > c->d.numberOfTimeSteps)
+      return fail("multi_run: steps [%d,%d) outside [0,%d) of a variant", ts, ts + count, c->d.numberOfTimeSteps);
+  HIP_TRY(hipSetDevice(m->device));
+  if (multi_borrow(m)) return FDTD_HIP_ERR;
+  int rc = m->fp32 ? multi_prepare<float>(m) : multi_prepare<double>(m);
+  if (!rc) rc = m->fp32 ? multi_steps<float>(m, ts, count, graphs) : multi_steps<double>(m, ts, count, graphs);
+  const int rc2 = multi_return(m);
+  if (rc || rc2) return FDTD_HIP_ERR;
+  for (FdtdHipContext *c : m->ctx) c->steps_run += count;
+  return FDTD_HIP_OK;
+}
+
+}  // namespace
+}  // extern "C++"
+
+void fdtd_hip_multi_destroy(FdtdHipMulti *m) {
+  if (!m) return;
+  if (m->stream) (void)hipStreamSynchronize(m->stream);
+  multi_drop_graph(m);
+  (void)hipFree(m->d_desc);
+  (void)hipFree(m->d_map);
+  (void)hipFree(m->d_step);
+  if (m->ev) (void)hipEventDestroy(m->ev);
+  if (m->stream) (void)hipStreamDestroy(m->stream);
+  delete m;
+}
+
+int fdtd_hip_multi_create(FdtdHipContext *const *ctxs, int n, const FdtdHipTuning *tuning, FdtdHipMulti **out) {
+  if (!ctxs || !out) return fail("multi_create: null argument");
+  *out = nullptr;
+  if (n < 1 || n > MULTI_MAX_VARIANTS) return fail("multi_create: %d variants (1..%d)", n, MULTI_MAX_VARIANTS);
+  for (int v = 0; v < n; ++v) {
+    if (!ctxs[v]) return fail("multi_create: variant %d is null", v);
+    for (int u = 0; u < v; ++u)
+      if (ctxs[u] == ctxs[v]) return fail("multi_create: variant %d repeats variant %d", v, u);
+    if (ctxs[v]->device != ctxs[0]->device) return fail("multi_create: variants on different devices");
+    if (ctxs[v]->fp32() != ctxs[0]->fp32()) return fail("multi_create: variants mix FP32 and FP64");
+  }
+  FdtdHipTuning t{0, 0, 0, CPML_FUSE_ALL};
+  if (tuning) {
+    t = *tuning;
+    if (t.fuse_cpml != 0 && t.fuse_cpml != CPML_FUSE_ALL)
+      return fail("multi_create: batched launches fuse every CPML face (fuse_cpml 0 or 2)");
+    t.fuse_cpml = CPML_FUSE_ALL;
+    if (check_tuning(ctxs[0], &t, "multi_create")) return FDTD_HIP_ERR;
+  }
+  HIP_TRY(hipSetDevice(ctxs[0]->device));
+  FdtdHipMulti *m = new FdtdHipMulti();
+  m->ctx.assign(ctxs, ctxs + n);
+  m->device = ctxs[0]->device;
+  m->fp32 = ctxs[0]->fp32();
+  m->tune = t;
+  if (hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&m->ev, hipEventDisableTiming) != hipSuccess ||
+      hipMalloc((void **)&m->d_step, sizeof(int)) != hipSuccess) {
+    fdtd_hip_multi_destroy(m);
+    return fail("multi_create: stream, event or step counter allocation failed");
+  }
+  *out = m;
+  return FDTD_HIP_OK;
+}
+
+int fdtd_hip_multi_run_async(FdtdHipMulti *m, int tsStart, int count) {
+  if (!m) return fail("multi_run: null batch");
+  return multi_run(m, tsStart, count, m->use_graphs != 0);
+}
+
+int fdtd_hip_multi_set_graphs(FdtdHipMulti *m, int enable) {
+  if (!m) return fail("multi_set_graphs: null batch");
+  m->use_graphs = enable ? 1 : 0;
+  return FDTD_HIP_OK;
+}
+
+int fdtd_hip_multi_synchronize(FdtdHipMulti *m) {
+  if (!m) return fail("multi_synchronize: null batch");
+  HIP_TRY(hipStreamSynchronize(m->stream));
+  return FDTD_HIP_OK;
+}
+
+int fdtd_hip_multi_get_tuning(FdtdHipMulti *m, FdtdHipTuning *out) {
+  if (!m || !out) return fail("multi_get_tuning: null argument");
+  *out = m->tune;
+  return FDTD_HIP_OK;
+}
+
+int fdtd_hip_multi_autotune(FdtdHipMulti *m, int steps_per_candidate, FdtdHipTuning *chosen) {
+  if (!m) return fail("multi_autotune: null batch");
+  for (FdtdHipContext *c : m->ctx)
+    if (c->steps_run) return fail("multi_autotune: must run before the first time step (it resets the state)");
+  if (steps_per_candidate < 1) steps_per_candidate = 2;
+  int steps = steps_per_candidate + 1;
+  for (FdtdHipContext *c : m->ctx) steps = std::min(steps, c->d.numberOfTimeSteps);
+  std::vector<FdtdHipTuning> cands;
+  for (int cs = 0; cs < 2; ++cs)
+    for (int w : {1, 2, 4}) {
+      if (w == 4 && !m->fp32) continue;
+      for (int pl : {1, 2, 4, 8}) cands.push_back(FdtdHipTuning{pl, w, cs, CPML_FUSE_ALL});
+    }
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  HIP_TRY(hipSetDevice(m->device));
+  HIP_TRY(hipEventCreate(&e0));
+  HIP_TRY(hipEventCreate(&e1));
+  double best = 1e300;
+  FdtdHipTuning best_t = m->tune;
+  int rc = FDTD_HIP_OK;
+  for (const auto &t : cands) {
+    m->tune = t;
+    // Step 0 warms up; steps 1.. are timed.
+    if ((rc = multi_run(m, 0, 1, false))) break;
+    if (hipEventRecord(e0, m->stream) != hipSuccess) { rc = fail("multi_autotune: event record failed"); break; }
+    if ((rc = multi_run(m, 1, steps - 1, false))) break;
+    float ms = 0.f;
+    if (hipEventRecord(e1, m->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+        hipEventElapsedTime(&ms, e0, e1) != hipSuccess) { rc = fail("multi_autotune: event timing failed"); break; }
+    if (ms < best) { best = ms; best_t = t; }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  m->tune = best_t;
+  for (FdtdHipContext *c : m->ctx) {
+    c->steps_run = 0;
+    if (!rc && reset_state(c)) rc = FDTD_HIP_ERR;
+  }
+  if (rc) return rc;
+  if (chosen) *chosen = best_t;
+  return FDTD_HIP_OK;
+}
 
 // ------------------------------------------------------------ early-stop network --
 

@@ -59,9 +59,12 @@ struct TableCache {
 template <typename T, int CM>
 __device__ __forceinline__ void load_tables(const Grid<T> &g, int first, TableCache<T, CM> &tc) {
   if (CM != COEF_U8) return;
-  for (int q = threadIdx.x; q < 3 * 3 * U8_TABLE; q += blockDim.x) {
-    const int comp = q / (3 * U8_TABLE), r = q - comp * (3 * U8_TABLE);
-    tc.v[q] = r < 3 * g.ct.n[first + comp] ? ((const T *)g.ct.tab[first + comp])[r] : T(0);
+  // Components unrolled so every table index is a constant (the Grid stays in registers).
+#pragma unroll
+  for (int comp = 0; comp < 3; ++comp) {
+    const T *tab = (const T *)g.ct.tab[first + comp];
+    const int n3 = 3 * g.ct.n[first + comp];
+    for (int r = threadIdx.x; r < 3 * U8_TABLE; r += blockDim.x) tc.v[comp * 3 * U8_TABLE + r] = r < n3 ? tab[r] : T(0);
   }
   __syncthreads();
 }
@@ -160,6 +163,82 @@ __device__ __forceinline__ int psi_offset(int i, int j, int k, int ci, int nc, i
   return (i * d1 + j) * nc + ci;
 }
 
+// Slot (0..nc-1) of face axis AXIS that (i,j,k) falls in, for the n face (s = 0) or
+// the p face (s = 1); -1 when outside.
+template <typename T, int AXIS, bool MAG>
+__device__ __forceinline__ int cpml_slot(const FaceSet<T> &fs, int s, int i, int j, int k) {
+  const Face<T> &F = fs.f[2 * AXIS + s];
+  const int coord = AXIS == 0 ? i : (AXIS == 1 ? j : k);
+  const int ci = coord - (MAG ? F.m_start : F.e_start);
+  return (unsigned)ci < (unsigned)F.nc ? ci : -1;
+}
+
+// Psi recurrence and field correction of one face slot (cpml.js:470,484 order).
+template <typename T, int AXIS, int SLOT, bool MAG>
+__device__ __forceinline__ T cpml_step(const FaceSet<T> &fs, int s, int ci, T val, T diff,
+                                       int i, int j, int k, int d1, int d2) {
+  const Face<T> &F = fs.f[2 * AXIS + s];
+  const int q = psi_offset<AXIS>(i, j, k, ci, F.nc, d1, d2);
+  const T b = MAG ? F.b_m[ci] : F.b_e[ci];
+  const T a = MAG ? F.a_m[ci] : F.a_e[ci];
+  const T ps = b * F.psi[SLOT][q] + a * diff;
+  F.psi[SLOT][q] = ps;
+  return val + F.cpsi[SLOT][q] * ps;
+}
+
+// One component's CPML terms on the value v: faces of axis A1 (n then p) then of axis
+// A2, each with its curl difference computed on demand; *hit tells whether any face
+// applied.
+template <typename T, int A1, int S1, int A2, int S2, bool MAG, typename D1, typename D2>
+__device__ __forceinline__ T cpml_terms(const FaceSet<T> &fs, T v, int i, int j, int k, int d1, int d2,
+                                        D1 diff1, D2 diff2, bool *hit) {
+  const int a0 = cpml_slot<T, A1, MAG>(fs, 0, i, j, k), a1 = cpml_slot<T, A1, MAG>(fs, 1, i, j, k);
+  const int b0 = cpml_slot<T, A2, MAG>(fs, 0, i, j, k), b1 = cpml_slot<T, A2, MAG>(fs, 1, i, j, k);
+  *hit = a0 >= 0 || a1 >= 0 || b0 >= 0 || b1 >= 0;
+  if (a0 >= 0 || a1 >= 0) {
+    const T d = diff1();
+    if (a0 >= 0) v = cpml_step<T, A1, S1, MAG>(fs, 0, a0, v, d, i, j, k, d1, d2);
+    if (a1 >= 0) v = cpml_step<T, A1, S1, MAG>(fs, 1, a1, v, d, i, j, k, d1, d2);
+  }
+  if (b0 >= 0 || b1 >= 0) {
+    const T d = diff2();
+    if (b0 >= 0) v = cpml_step<T, A2, S2, MAG>(fs, 0, b0, v, d, i, j, k, d1, d2);
+    if (b1 >= 0) v = cpml_step<T, A2, S2, MAG>(fs, 1, b1, v, d, i, j, k, d1, d2);
+  }
+  return v;
+}
+
+// The same on the stored element F[c]; loads and stores it only if a face applies.
+template <typename T, int A1, int S1, int A2, int S2, bool MAG, typename D1, typename D2>
+__device__ __forceinline__ void cpml_component(const FaceSet<T> &fs, T *__restrict__ F, int c,
+                                               int i, int j, int k, int d1, int d2, D1 diff1, D2 diff2) {
+  const int a0 = cpml_slot<T, A1, MAG>(fs, 0, i, j, k), a1 = cpml_slot<T, A1, MAG>(fs, 1, i, j, k);
+  const int b0 = cpml_slot<T, A2, MAG>(fs, 0, i, j, k), b1 = cpml_slot<T, A2, MAG>(fs, 1, i, j, k);
+  if (a0 < 0 && a1 < 0 && b0 < 0 && b1 < 0) return;
+  bool hit;
+  F[c] = cpml_terms<T, A1, S1, A2, S2, MAG>(fs, F[c], i, j, k, d1, d2, diff1, diff2, &hit);
+}
+
+// Whether coordinate v along AXIS lies in a slab of that axis's faces (MAG: H planes,
+// else E planes).
+template <typename T, int AXIS, bool MAG>
+__device__ __forceinline__ bool in_slab(const FaceSet<T> &fs, int v) {
+  return cpml_slot<T, AXIS, MAG>(fs, 0, v, v, v) >= 0 || cpml_slot<T, AXIS, MAG>(fs, 1, v, v, v) >= 0;
+}
+
+// CPML placement (a tuning choice; results are identical): the separate k_cpml pass
+// for every face, the z-face slabs fused into the bulk kernels, or every face fused.
+constexpr int CPML_SEPARATE = 0, CPML_FUSE_Z = 1, CPML_FUSE_ALL = 2;
+
+// Whether the bulk kernel applies the CPML terms of (i,j,k) itself.
+template <typename T, int FC, bool MAG>
+__device__ __forceinline__ bool fused_cpml_cell(const FaceSet<T> &fs, int i, int j, int k) {
+  if (FC == CPML_FUSE_Z) return in_slab<T, 2, MAG>(fs, k);
+  if (FC == CPML_FUSE_ALL)
+    return in_slab<T, 0, MAG>(fs, i) || in_slab<T, 1, MAG>(fs, j) || in_slab<T, 2, MAG>(fs, k);
+  return false;
+}
+
 // Launch geometry shared by the two stencil kernels.  A thread updates W consecutive
 // cells of one row (rows are padded to a multiple of 4 elements, so the W cells never
 // straddle a row); a workgroup's 256 threads cover 256*W consecutive plane offsets
@@ -196,16 +275,17 @@ struct Fields {
 };
 
 // H bulk update; W consecutive cells of a row per thread.  Hx over i in [0,nx], Hy/Hz
-// over i in [0,nx).  The CPML terms are added afterwards by k_cpml (the reference also
-// applies them after the whole bulk update, js_fdtd/src/fdtdWorker.js:62-79).  Every
-// array has a zero guard plane before plane 0 and after plane nx, so all neighbour
-// loads are unconditional; elements outside a component's Yee shape are written back
-// unchanged.
-template <typename T, int CM, int W>
+// over i in [0,nx).  The reference applies the CPML terms after the whole bulk update
+// (js_fdtd/src/fdtdWorker.js:62-79); a cell's terms depend only on its own bulk value,
+// its psi and E, so applying them to the cell right after its bulk update (FC != 0)
+// gives the same result as the separate k_cpml pass.  Every array has a zero guard
+// plane before plane 0 and after plane nx, so all neighbour loads are unconditional;
+// elements outside a component's Yee shape are written back unchanged.
+template <typename T, int CM, int W, int FC>
 __device__ __forceinline__ void update_h_body(
     T *__restrict__ Hx, T *__restrict__ Hy, T *__restrict__ Hz,
     const T *__restrict__ Ex, const T *__restrict__ Ey, const T *__restrict__ Ez,
-    const Grid<T> &g, const MarchGeom &m, const TableCache<T, CM> &tc, int col, int ch) {
+    const Grid<T> &g, const MarchGeom &m, const TableCache<T, CM> &tc, int col, int ch, const FaceSet<T> &fs) {
 #pragma clang fp contract(off)
   typedef Vec<T, W> V;
   const int p = (col * 256 + (int)threadIdx.x) * W;
@@ -249,6 +329,25 @@ __device__ __forceinline__ void update_h_body(
         hz[e] = h;
       }
     }
+    if (FC != CPML_SEPARATE) {
+      // Cells the bulk kernel owns get all their CPML terms here, right after their
+      // bulk update and in the reference's face order; k_cpml skips them.
+#pragma unroll
+      for (int e = 0; e < W; ++e) {
+        const int k = k0 + e, ce = c + e;
+        if (k > nz || !fused_cpml_cell<T, FC, true>(fs, i, j, k)) continue;
+        bool hit;
+        if (j < ny && k < nz)                              // Hx: y then z faces
+          hx[e] = cpml_terms<T, 1, 0, 2, 0, true>(fs, hx[e], i, j, k, ny, nz,
+              [&] { return Ez[ce + SJ] - Ez[ce]; }, [&] { return Ey[ce + 1] - Ey[ce]; }, &hit);
+        if (inner && k < nz)                               // Hy: x then z faces
+          hy[e] = cpml_terms<T, 0, 0, 2, 1, true>(fs, hy[e], i, j, k, ny + 1, nz,
+              [&] { return Ez[ce + SI] - Ez[ce]; }, [&] { return Ex[ce + 1] - Ex[ce]; }, &hit);
+        if (inner && j < ny)                               // Hz: x then y faces
+          hz[e] = cpml_terms<T, 0, 1, 1, 1, true>(fs, hz[e], i, j, k, ny, nz + 1,
+              [&] { return Ey[ce + SI] - Ey[ce]; }, [&] { return Ex[ce + SJ] - Ex[ce]; }, &hit);
+      }
+    }
     stv<T, W>(Hx + c, hx);
     if (inner) { stv<T, W>(Hy + c, hy); stv<T, W>(Hz + c, hz); }
     ey_i = ey_n;
@@ -256,13 +355,13 @@ __device__ __forceinline__ void update_h_body(
   }
 }
 
-template <typename T, int CM, int W>
-__global__ void __launch_bounds__(256) k_update_h(Grid<T> g, MarchGeom m) {
+template <typename T, int CM, int W, int FC>
+__global__ void __launch_bounds__(256) k_update_h(Grid<T> g, MarchGeom m, FaceSet<T> fs) {
   __shared__ TableCache<T, CM> tc;
   int col, ch;
   if (!march_tile(m, col, ch)) return;
   load_tables<T, CM>(g, 0, tc);
-  update_h_body<T, CM, W>(g.Hx, g.Hy, g.Hz, g.Ex, g.Ey, g.Ez, g, m, tc, col, ch);
+  update_h_body<T, CM, W, FC>(g.Hx, g.Hy, g.Hz, g.Ex, g.Ey, g.Ez, g, m, tc, col, ch, fs);
 }
 
 // E bulk update; W consecutive cells of a row per thread.  With ghost != 0 (reference
@@ -270,13 +369,15 @@ __global__ void __launch_bounds__(256) k_update_h(Grid<T> g, MarchGeom m) {
 // the i = nx wall the way the reference's updateE does: Ey there differences
 // "Hz[nx]", which is Ex(0,j,k) after this step's bulk update (the CPML terms come
 // later, in k_cpml), and Ez differences "Hy[nx]", which is the reference-layout flat
-// element j*nz+k of Hz.  With ghost == 0 the wall is left to the CPML terms.
-// Elements the bulk does not update keep their value.
-template <typename T, int CM, int W>
+// element j*nz+k of Hz.  With ghost == 0 the wall is left to the CPML terms.  With
+// every face fused (FC == CPML_FUSE_ALL) the same thread then applies the wall's CPML
+// terms, which read only H.  Elements the bulk does not update keep their value.
+template <typename T, int CM, int W, int FC>
 __device__ __forceinline__ void update_e_body(
     T *__restrict__ Ex, T *__restrict__ Ey, T *__restrict__ Ez,
     const T *__restrict__ Hx, const T *__restrict__ Hy, const T *__restrict__ Hz,
-    const Grid<T> &g, const MarchGeom &m, const TableCache<T, CM> &tc, int col, int ch, int ghost) {
+    const Grid<T> &g, const MarchGeom &m, const TableCache<T, CM> &tc, int col, int ch, int ghost,
+    const FaceSet<T> &fs) {
 #pragma clang fp contract(off)
   typedef Vec<T, W> V;
   const int p = (col * 256 + (int)threadIdx.x) * W;
@@ -329,6 +430,24 @@ __device__ __forceinline__ void update_e_body(
         ez[e] = v;
       }
     }
+    if (FC != CPML_SEPARATE) {
+      // As in update_h_body; exb keeps the pre-CPML Ex that the ghost plane reads.
+#pragma unroll
+      for (int e = 0; e < W; ++e) {
+        const int k = k0 + e, ce = c + e;
+        if (k > nz || !fused_cpml_cell<T, FC, false>(fs, i, j, k)) continue;
+        bool hit;
+        if (i < nx)                                        // Ex: y then z faces
+          ex[e] = cpml_terms<T, 1, 3, 2, 2, false>(fs, ex[e], i, j, k, ny + 1, nz + 1,
+              [&] { return Hz[ce] - Hz[ce - SJ]; }, [&] { return Hy[ce] - Hy[ce - 1]; }, &hit);
+        if (j < ny)                                        // Ey: x then z faces
+          ey[e] = cpml_terms<T, 0, 2, 2, 3, false>(fs, ey[e], i, j, k, ny, nz + 1,
+              [&] { return Hz[ce] - Hz[ce - SI]; }, [&] { return Hx[ce] - Hx[ce - 1]; }, &hit);
+        if (k < nz)                                        // Ez: x then y faces
+          ez[e] = cpml_terms<T, 0, 3, 1, 2, false>(fs, ez[e], i, j, k, ny + 1, nz,
+              [&] { return Hy[ce] - Hy[ce - SI]; }, [&] { return Hx[ce] - Hx[ce - SJ]; }, &hit);
+      }
+    }
     if (i < nx) stv<T, W>(Ex + c, ex);
     stv<T, W>(Ey + c, ey);
     stv<T, W>(Ez + c, ez);
@@ -355,18 +474,33 @@ __device__ __forceinline__ void update_e_body(
         }
       }
     }
+    if (FC == CPML_FUSE_ALL && i == 0) {
+      const int cg = nx * SI + p;
+#pragma unroll
+      for (int e = 0; e < W; ++e) {
+        const int k = k0 + e, ce = cg + e;
+        if (k > nz || !fused_cpml_cell<T, FC, false>(fs, nx, j, k)) continue;
+        bool hit;
+        if (j < ny)                                        // Ey(nx): x then z faces
+          Ey[ce] = cpml_terms<T, 0, 2, 2, 3, false>(fs, Ey[ce], nx, j, k, ny, nz + 1,
+              [&] { return Hz[ce] - Hz[ce - SI]; }, [&] { return Hx[ce] - Hx[ce - 1]; }, &hit);
+        if (k < nz)                                        // Ez(nx): x then y faces
+          Ez[ce] = cpml_terms<T, 0, 3, 1, 2, false>(fs, Ez[ce], nx, j, k, ny + 1, nz,
+              [&] { return Hy[ce] - Hy[ce - SI]; }, [&] { return Hx[ce] - Hx[ce - SJ]; }, &hit);
+      }
+    }
     hz_m = hz_i;
     hy_m = hy_i;
   }
 }
 
-template <typename T, int CM, int W>
-__global__ void __launch_bounds__(256) k_update_e(Grid<T> g, MarchGeom m, int ghost) {
+template <typename T, int CM, int W, int FC>
+__global__ void __launch_bounds__(256) k_update_e(Grid<T> g, MarchGeom m, int ghost, FaceSet<T> fs) {
   __shared__ TableCache<T, CM> tc;
   int col, ch;
   if (!march_tile(m, col, ch)) return;
   load_tables<T, CM>(g, 3, tc);
-  update_e_body<T, CM, W>(g.Ex, g.Ey, g.Ez, g.Hx, g.Hy, g.Hz, g, m, tc, col, ch, ghost);
+  update_e_body<T, CM, W, FC>(g.Ex, g.Ey, g.Ez, g.Hx, g.Hy, g.Hz, g, m, tc, col, ch, ghost, fs);
 }
 
 // ------------------------------------------------------------------ CPML pass --
@@ -380,52 +514,6 @@ struct CpmlBoxes {
   long long start[MAX_CPML_BOXES + 1];   // flat start of each box; start[nbox] = total cells
   int lo[MAX_CPML_BOXES][3], ext[MAX_CPML_BOXES][3];
 };
-
-// Slot (0..nc-1) of face axis AXIS that (i,j,k) falls in, for the n face (s = 0) or
-// the p face (s = 1); -1 when outside.
-template <typename T, int AXIS, bool MAG>
-__device__ __forceinline__ int cpml_slot(const FaceSet<T> &fs, int s, int i, int j, int k) {
-  const Face<T> &F = fs.f[2 * AXIS + s];
-  const int coord = AXIS == 0 ? i : (AXIS == 1 ? j : k);
-  const int ci = coord - (MAG ? F.m_start : F.e_start);
-  return (unsigned)ci < (unsigned)F.nc ? ci : -1;
-}
-
-// Psi recurrence and field correction of one face slot (cpml.js:470,484 order).
-template <typename T, int AXIS, int SLOT, bool MAG>
-__device__ __forceinline__ T cpml_step(const FaceSet<T> &fs, int s, int ci, T val, T diff,
-                                       int i, int j, int k, int d1, int d2) {
-  const Face<T> &F = fs.f[2 * AXIS + s];
-  const int q = psi_offset<AXIS>(i, j, k, ci, F.nc, d1, d2);
-  const T b = MAG ? F.b_m[ci] : F.b_e[ci];
-  const T a = MAG ? F.a_m[ci] : F.a_e[ci];
-  const T ps = b * F.psi[SLOT][q] + a * diff;
-  F.psi[SLOT][q] = ps;
-  return val + F.cpsi[SLOT][q] * ps;
-}
-
-// One component's CPML terms: faces of axis A1 (n then p) then of axis A2, each with
-// its curl difference computed on demand.  Loads and stores the field only if a face
-// applies.
-template <typename T, int A1, int S1, int A2, int S2, bool MAG, typename D1, typename D2>
-__device__ __forceinline__ void cpml_component(const FaceSet<T> &fs, T *__restrict__ F, int c,
-                                               int i, int j, int k, int d1, int d2, D1 diff1, D2 diff2) {
-  const int a0 = cpml_slot<T, A1, MAG>(fs, 0, i, j, k), a1 = cpml_slot<T, A1, MAG>(fs, 1, i, j, k);
-  const int b0 = cpml_slot<T, A2, MAG>(fs, 0, i, j, k), b1 = cpml_slot<T, A2, MAG>(fs, 1, i, j, k);
-  if (a0 < 0 && a1 < 0 && b0 < 0 && b1 < 0) return;
-  T v = F[c];
-  if (a0 >= 0 || a1 >= 0) {
-    const T d = diff1();
-    if (a0 >= 0) v = cpml_step<T, A1, S1, MAG>(fs, 0, a0, v, d, i, j, k, d1, d2);
-    if (a1 >= 0) v = cpml_step<T, A1, S1, MAG>(fs, 1, a1, v, d, i, j, k, d1, d2);
-  }
-  if (b0 >= 0 || b1 >= 0) {
-    const T d = diff2();
-    if (b0 >= 0) v = cpml_step<T, A2, S2, MAG>(fs, 0, b0, v, d, i, j, k, d1, d2);
-    if (b1 >= 0) v = cpml_step<T, A2, S2, MAG>(fs, 1, b1, v, d, i, j, k, d1, d2);
-  }
-  F[c] = v;
-}
 
 __device__ __forceinline__ bool cpml_cell(const CpmlBoxes &bx, long long q, int &i, int &j, int &k) {
   if (q >= bx.start[bx.nbox]) return false;
@@ -519,12 +607,12 @@ __device__ __forceinline__ double ordered_wave_sum(double acc, double mine, int 
 // of the first step of a replayed graph plus the kernel's offset within the graph.
 __device__ __forceinline__ int step_of(const int *tsp, int off) { return off + (tsp ? *tsp : 0); }
 
-// Source injection, then port voltages and currents, then their running DFTs.
-// One block; the injection precedes the captures as in fdtdSolver.js:111-117.
+// Source injection, then port voltages and currents, then their running DFTs, for
+// step ts; run by one whole block.  The injection precedes the captures as in
+// fdtdSolver.js:111-117.
 template <typename T>
-__global__ void __launch_bounds__(256) k_port_epilogue(Grid<T> g, PortState ps, const int *tsp, int off) {
+__device__ __forceinline__ void port_epilogue_body(const Grid<T> &g, const PortState &ps, int ts) {
 #pragma clang fp contract(off)
-  const int ts = step_of(tsp, off);
   __shared__ double s_val[2 * MAX_PORT_OBJS];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nwaves = blockDim.x >> 6;
   T *E[3] = {g.Ex, g.Ey, g.Ez};
@@ -579,6 +667,11 @@ __global__ void __launch_bounds__(256) k_port_epilogue(Grid<T> g, PortState ps, 
   }
 }
 
+template <typename T>
+__global__ void __launch_bounds__(256) k_port_epilogue(Grid<T> g, PortState ps, const int *tsp, int off) {
+  port_epilogue_body<T>(g, ps, step_of(tsp, off));
+}
+
 // ------------------------------------------------------------------------ NTFF --
 
 struct FarfieldState {
@@ -602,10 +695,8 @@ __host__ __device__ __forceinline__ long ff_offset(int a, int nfreq, long sx, lo
 // all frequencies in a loop.  Face and component mapping follow farfield.js:115-246
 // (including the zp-face Jx subtract-then-add sequence, :210-226).
 template <typename T>
-__global__ void __launch_bounds__(256) k_ntff_direct(Grid<T> g, FarfieldState fs, const int *tsp, int off) {
+__device__ __forceinline__ void ntff_direct_body(const Grid<T> &g, const FarfieldState &fs, int ts, long q) {
 #pragma clang fp contract(off)
-  const int ts = step_of(tsp, off);
-  const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
   const long sx = (long)fs.nj * fs.nk, sy = (long)fs.ni * fs.nk, sz = (long)fs.ni * fs.nj;
   const long total = 2 * (sx + sy + sz);
   if (q >= total) return;
@@ -688,6 +779,11 @@ __global__ void __launch_bounds__(256) k_ntff_direct(Grid<T> g, FarfieldState fs
     }
     rJ1[pos] = rJ1[pos] + ch * j1; iJ1[pos] = iJ1[pos] + sh * j1;
   }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_ntff_direct(Grid<T> g, FarfieldState fs, const int *tsp, int off) {
+  ntff_direct_body<T>(g, fs, step_of(tsp, off), (long)blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 // Advances the device-held step by `by` at the end of a replayed graph.

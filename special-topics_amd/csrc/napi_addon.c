@@ -18,7 +18,7 @@
  *   uploadPortDft(h, Float64Array freqHz)
  *   runBatch(h, tsStart, count, Float64Array|null sv, Float64Array|null si)
  *   readField(h, which, Float64Array); writeField(h, which, Float64Array)
- *   setTuning(h, {planesPerWorkgroup, cellsPerThread, coefficientStorage})
+ *   setTuning(h, {planesPerWorkgroup, cellsPerThread, coefficientStorage, fuseCpml})
  *   getTuning(h) -> tuning; autotune(h, stepsPerCandidate) -> tuning
  *   readFarfield(h, which, Float64Array re, Float64Array im)
  *   readPortDft(h, Float64Array re, Float64Array im)
@@ -26,6 +26,11 @@
  *   directivity(h, freqHz, radPower|null, theta, phi, dataTheta, dataPhi)
  *   setTiming(h, bool); kernelTimes(h) -> {ms:[6], launches:[6]}
  *   destroy(h)
+ *   readTraces(h, tsStart, count, Float64Array|null sv, Float64Array|null si)
+ *   multiCreate([h, ...], tuning|null) -> batch handle   (fdtd_hip_multi_*: many
+ *   multiRun(m, tsStart, count); multiSynchronize(m)       problems, one launch per
+ *   multiAutotune(m, stepsPerCandidate) -> tuning          phase)
+ *   multiDestroy(m)
  */
 #define NAPI_VERSION 8
 #include <node_api.h>
@@ -431,11 +436,13 @@ static napi_value tuning_object(napi_env env, const FdtdHipTuning *t) {
   CALL(env, napi_set_named_property(env, obj, "cellsPerThread", x));
   CALL(env, napi_create_int32(env, t->coefficient_storage, &x));
   CALL(env, napi_set_named_property(env, obj, "coefficientStorage", x));
+  CALL(env, napi_create_int32(env, t->fuse_cpml, &x));
+  CALL(env, napi_set_named_property(env, obj, "fuseCpml", x));
   return obj;
 }
 
-/* setTuning(h, {planesPerWorkgroup, cellsPerThread, coefficientStorage}); absent or
- * zero members select the library defaults. */
+/* setTuning(h, {planesPerWorkgroup, cellsPerThread, coefficientStorage, fuseCpml});
+ * absent or zero members select the library defaults. */
 static napi_value SetTuning(napi_env env, napi_callback_info info) {
   napi_value argv[2];
   if (!get_args(env, info, 2, argv)) return NULL;
@@ -443,9 +450,9 @@ static napi_value SetTuning(napi_env env, napi_callback_info info) {
   if (!c) return NULL;
   FdtdHipTuning t;
   memset(&t, 0, sizeof t);
-  const char *keys[3] = {"planesPerWorkgroup", "cellsPerThread", "coefficientStorage"};
-  int *dst[3] = {&t.planes_per_workgroup, &t.cells_per_thread, &t.coefficient_storage};
-  for (int q = 0; q < 3; ++q) {
+  const char *keys[4] = {"planesPerWorkgroup", "cellsPerThread", "coefficientStorage", "fuseCpml"};
+  int *dst[4] = {&t.planes_per_workgroup, &t.cells_per_thread, &t.coefficient_storage, &t.fuse_cpml};
+  for (int q = 0; q < 4; ++q) {
     bool has = false;
     if (napi_has_named_property(env, argv[1], keys[q], &has) != napi_ok) { throw_napi(env, keys[q]); return NULL; }
     if (has && !prop_int(env, argv[1], keys[q], dst[q])) return NULL;
@@ -485,6 +492,122 @@ static napi_value Destroy(napi_env env, napi_callback_info info) {
   return undefined(env);
 }
 
+static napi_value ReadTraces(napi_env env, napi_callback_info info) {
+  napi_value argv[5];
+  int ts, count, ok;
+  if (!get_args(env, info, 5, argv)) return NULL;
+  FdtdHipContext *c = handle(env, argv[0]);
+  if (!c || !get_int(env, argv[1], &ts) || !get_int(env, argv[2], &count)) return NULL;
+  double *sv = typed(env, argv[3], napi_float64_array, NULL, 1, &ok);
+  if (!ok) return NULL;
+  double *si = typed(env, argv[4], napi_float64_array, NULL, 1, &ok);
+  if (!ok) return NULL;
+  if (fdtd_hip_read_traces(c, ts, count, sv, si) != FDTD_HIP_OK) return throw_lib(env, "readTraces");
+  return undefined(env);
+}
+
+/* Batch handles carry a type tag so a problem handle is never taken for one. */
+static const napi_type_tag kMultiTag = {0x6664746468697030ULL, 0x6d756c7469626174ULL};
+
+static FdtdHipMulti *multi_handle(napi_env env, napi_value v) {
+  bool is = false;
+  void *p = NULL;
+  if (napi_check_object_type_tag(env, v, &kMultiTag, &is) != napi_ok || !is ||
+      napi_get_value_external(env, v, &p) != napi_ok || !p) {
+    napi_throw_type_error(env, NULL, "expected a live batch handle");
+    return NULL;
+  }
+  return (FdtdHipMulti *)p;
+}
+
+static int tuning_from(napi_env env, napi_value obj, FdtdHipTuning *t) {
+  const char *keys[4] = {"planesPerWorkgroup", "cellsPerThread", "coefficientStorage", "fuseCpml"};
+  int *dst[4] = {&t->planes_per_workgroup, &t->cells_per_thread, &t->coefficient_storage, &t->fuse_cpml};
+  memset(t, 0, sizeof *t);
+  for (int q = 0; q < 4; ++q) {
+    bool has = false;
+    if (napi_has_named_property(env, obj, keys[q], &has) != napi_ok) { throw_napi(env, keys[q]); return 0; }
+    if (has && !prop_int(env, obj, keys[q], dst[q])) return 0;
+  }
+  return 1;
+}
+
+static napi_value MultiCreate(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  if (!get_args(env, info, 2, argv)) return NULL;
+  bool is_array = false;
+  uint32_t n = 0;
+  if (napi_is_array(env, argv[0], &is_array) != napi_ok || !is_array ||
+      napi_get_array_length(env, argv[0], &n) != napi_ok || n == 0) {
+    napi_throw_type_error(env, NULL, "expected a non-empty array of problem handles");
+    return NULL;
+  }
+  FdtdHipContext *ctx[4096];
+  if (n > 4096) { napi_throw_range_error(env, NULL, "at most 4096 problems per batch"); return NULL; }
+  for (uint32_t q = 0; q < n; ++q) {
+    napi_value e;
+    CALL(env, napi_get_element(env, argv[0], q, &e));
+    if (!(ctx[q] = handle(env, e))) return NULL;
+  }
+  napi_valuetype vt;
+  CALL(env, napi_typeof(env, argv[1], &vt));
+  FdtdHipTuning t;
+  const FdtdHipTuning *tp = NULL;
+  if (vt != napi_null && vt != napi_undefined) {
+    if (!tuning_from(env, argv[1], &t)) return NULL;
+    tp = &t;
+  }
+  FdtdHipMulti *m = NULL;
+  if (fdtd_hip_multi_create(ctx, (int)n, tp, &m) != FDTD_HIP_OK) return throw_lib(env, "multiCreate");
+  napi_value ext;
+  if (napi_create_external(env, m, NULL, NULL, &ext) != napi_ok ||
+      napi_type_tag_object(env, ext, &kMultiTag) != napi_ok) {
+    fdtd_hip_multi_destroy(m);
+    throw_napi(env, "multiCreate");
+    return NULL;
+  }
+  return ext;
+}
+
+static napi_value MultiRun(napi_env env, napi_callback_info info) {
+  napi_value argv[3];
+  int ts, count;
+  if (!get_args(env, info, 3, argv)) return NULL;
+  FdtdHipMulti *m = multi_handle(env, argv[0]);
+  if (!m || !get_int(env, argv[1], &ts) || !get_int(env, argv[2], &count)) return NULL;
+  if (fdtd_hip_multi_run_async(m, ts, count) != FDTD_HIP_OK) return throw_lib(env, "multiRun");
+  return undefined(env);
+}
+
+static napi_value MultiSynchronize(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  if (!get_args(env, info, 1, argv)) return NULL;
+  FdtdHipMulti *m = multi_handle(env, argv[0]);
+  if (!m) return NULL;
+  if (fdtd_hip_multi_synchronize(m) != FDTD_HIP_OK) return throw_lib(env, "multiSynchronize");
+  return undefined(env);
+}
+
+static napi_value MultiAutotune(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  int steps;
+  if (!get_args(env, info, 2, argv)) return NULL;
+  FdtdHipMulti *m = multi_handle(env, argv[0]);
+  if (!m || !get_int(env, argv[1], &steps)) return NULL;
+  FdtdHipTuning t;
+  if (fdtd_hip_multi_autotune(m, steps, &t) != FDTD_HIP_OK) return throw_lib(env, "multiAutotune");
+  return tuning_object(env, &t);
+}
+
+static napi_value MultiDestroy(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  if (!get_args(env, info, 1, argv)) return NULL;
+  FdtdHipMulti *m = multi_handle(env, argv[0]);
+  if (!m) return NULL;
+  fdtd_hip_multi_destroy(m);
+  return undefined(env);
+}
+
 static napi_value Register(napi_env env, napi_value exports) {
   napi_property_descriptor d[] = {
     {"deviceCount", NULL, DeviceCount, NULL, NULL, NULL, napi_enumerable, NULL},
@@ -509,6 +632,12 @@ static napi_value Register(napi_env env, napi_value exports) {
     {"setTiming", NULL, SetTiming, NULL, NULL, NULL, napi_enumerable, NULL},
     {"kernelTimes", NULL, KernelTimes, NULL, NULL, NULL, napi_enumerable, NULL},
     {"destroy", NULL, Destroy, NULL, NULL, NULL, napi_enumerable, NULL},
+    {"readTraces", NULL, ReadTraces, NULL, NULL, NULL, napi_enumerable, NULL},
+    {"multiCreate", NULL, MultiCreate, NULL, NULL, NULL, napi_enumerable, NULL},
+    {"multiRun", NULL, MultiRun, NULL, NULL, NULL, napi_enumerable, NULL},
+    {"multiSynchronize", NULL, MultiSynchronize, NULL, NULL, NULL, napi_enumerable, NULL},
+    {"multiAutotune", NULL, MultiAutotune, NULL, NULL, NULL, napi_enumerable, NULL},
+    {"multiDestroy", NULL, MultiDestroy, NULL, NULL, NULL, napi_enumerable, NULL},
   };
   if (napi_define_properties(env, exports, sizeof d / sizeof d[0], d) != napi_ok) {
     throw_napi(env, "define_properties");

@@ -33,9 +33,9 @@ struct NtffBlock {
 // Gather: the four tangential values of every face cell of this step into row `row`.
 // Face and component mapping as k_ntff_direct (farfield.js:115-246).
 template <typename T>
-__global__ void __launch_bounds__(256) k_ntff_gather(Grid<T> g, FarfieldState fs, NtffBlock<T> nb, int row) {
+__device__ __forceinline__ void ntff_gather_body(const Grid<T> &g, const FarfieldState &fs, const NtffBlock<T> &nb,
+                                                 int row, long q) {
 #pragma clang fp contract(off)
-  const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
   const long sx = (long)fs.nj * fs.nk, sy = (long)fs.ni * fs.nk, sz = (long)fs.ni * fs.nj;
   if (q >= 2 * (sx + sy + sz)) return;
   const int SJ = g.SJ, SI = g.SI;
@@ -82,6 +82,11 @@ __global__ void __launch_bounds__(256) k_ntff_gather(Grid<T> g, FarfieldState fs
   r[nb.base[a0 + 1] + cell] = tJ1;
   r[nb.base[a0 + 2] + cell] = tM0;
   r[nb.base[a0 + 3] + cell] = tM1;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_ntff_gather(Grid<T> g, FarfieldState fs, NtffBlock<T> nb, int row) {
+  ntff_gather_body<T>(g, fs, nb, row, (long)blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 // Fold rows [r0, r1) of the block starting at step ts0 (see step_of) into the accumulators of

@@ -16,6 +16,11 @@ tail reproduces the full-rate DFT sum to about 0.02 dB (midpoint rule, phase ste
 
     python -m fdtdhip.earlystop gen-train --first 0 --count 2000 --out train0.npz
     python -m fdtdhip.earlystop gen-heldout --first 4000 --count 500 --out heldout.npz
+    python -m fdtdhip.earlystop evaluate --weights W.npz --heldout heldout.npz
+    python -m fdtdhip.earlystop tts-batched --weights W.npz --first 4600 --count 64
+
+Data generation and the batched time-to-solution step `--batch` variants together in
+one batched launch per phase (fdtdhip.sim.HipMulti).
 """
 
 from __future__ import annotations
@@ -28,7 +33,7 @@ import time
 
 import numpy as np
 
-from .sweep import LazyBuilds, Sweep, frequency_grid
+from .sweep import LazyBuilds, Sweep, build_all, frequency_grid
 
 N_PREFIX = 1500
 POOL_IN = 4
@@ -115,12 +120,13 @@ def _scenarios(first, count, seed, workdir, steps=STEPS):
     return paths, np.array(params)
 
 
-def gen_train(first, count, out, seed=4, workdir="/tmp/es-data", inflight=8, jobs=16):
-    """FP32 full runs (matlab boundary) of variants [first, first+count): input, target."""
+def gen_train(first, count, out, seed=4, workdir="/tmp/es-data", batch=64, jobs=16):
+    """FP32 full runs (matlab boundary) of variants [first, first+count), `batch` at a
+    time in one batched launch: input, target."""
     paths, params = _scenarios(first, count, seed, workdir)
-    builds = LazyBuilds(paths, workdir, fp32=True, jobs=jobs)
-    res, secs, cells = Sweep("fp32", "matlab", inflight, tuning=(1, 4, 0), keep_traces=True,
-                             farfield=False, s11=False, progress_every=100).run(builds)
+    builds = LazyBuilds(paths, workdir, fp32=True, jobs=jobs, ahead=2 * batch)
+    res, secs, cells = Sweep("fp32", "matlab", keep_traces=True, farfield=False, s11=False,
+                             progress_every=1, batch=batch).run(builds)
     builds.close()
     xs, scales, tails, logr, dims, dts = [], [], [], [], [], []
     for r in res:
@@ -133,17 +139,17 @@ def gen_train(first, count, out, seed=4, workdir="/tmp/es-data", inflight=8, job
     return {"variants": count, "run_s": round(secs, 1), "mcell_updates_per_s": round(cells / secs / 1e6, 1)}
 
 
-def gen_heldout(first, count, out, seed=4, workdir="/tmp/es-heldout", inflight=8, jobs=16):
+def gen_heldout(first, count, out, seed=4, workdir="/tmp/es-heldout", batch=64, jobs=16):
     """Per held-out variant: the full-length FP64 S11 (the judge) and the FP32 prefix
     traces at full rate (what an early-stopped run sees)."""
     paths, params = _scenarios(first, count, seed, workdir)
-    b64 = LazyBuilds(paths, os.path.join(workdir, "f64"), fp32=False, jobs=jobs)
-    res64, secs64, cells64 = Sweep("fp64", "matlab", inflight, keep_traces=False, farfield=False,
-                                   progress_every=50).run(b64)
+    b64 = LazyBuilds(paths, os.path.join(workdir, "f64"), fp32=False, jobs=jobs, ahead=2 * batch)
+    res64, secs64, cells64 = Sweep("fp64", "matlab", keep_traces=False, farfield=False,
+                                   progress_every=1, batch=batch).run(b64)
     b64.close()
-    b32 = LazyBuilds(paths, os.path.join(workdir, "f32"), fp32=True, jobs=jobs)
-    res32, secs32, _ = Sweep("fp32", "matlab", inflight, tuning=(1, 4, 0), keep_traces=True, farfield=False,
-                             s11=False, steps=N_PREFIX, progress_every=100).run(b32)
+    b32 = LazyBuilds(paths, os.path.join(workdir, "f32"), fp32=True, jobs=jobs, ahead=2 * batch)
+    res32, secs32, _ = Sweep("fp32", "matlab", keep_traces=True, farfield=False, s11=False, steps=N_PREFIX,
+                             progress_every=1, batch=batch).run(b32)
     b32.close()
     np.savez_compressed(out, index=np.arange(first, first + count), params=params,
                         s11_fp64=np.array([r["s11"] for r in res64]),
@@ -275,26 +281,95 @@ def time_to_solution(weights_path, first, count, seed=4, workdir="/tmp/es-tts"):
             "max_err_db": round(max(r["max_err_db"] for r in rows), 4)}
 
 
+def time_to_solution_batched(weights_path, first, count, seed=4, workdir="/tmp/es-tts-batch", jobs=16):
+    """The same comparison with every variant in one batched launch on each side: the
+    FP64 judge runs all `count` variants for 7000 steps; the early-stopped path runs
+    them in FP32 for 1500 steps, evaluates the network on the whole batch and forms
+    S11.  A fallback costs its share (1/count) of the FP64 batch.  Builds, uploads and
+    the batch autotuner run before each clock starts; times are per variant."""
+    from .problem import read_problem
+    from .sim import HipMulti, HipSim
+    from .sweep import s11
+    paths, _ = _scenarios(first, count, seed, workdir)
+    freqs = frequency_grid({"maxFreq": 6e9, "step": 6e6})
+    p64 = [read_problem(f) for f in build_all(paths, os.path.join(workdir, "f64"), fp32=False, jobs=jobs)]
+    p32 = [read_problem(f) for f in build_all(paths, os.path.join(workdir, "f32"), fp32=True, jobs=jobs)]
+    for p in p64:
+        _tail_phases(p.grid["dt"], freqs)
+
+    sims = [HipSim(p, precision="fp64", boundary="matlab", port_dft_freqs=freqs, farfield=False) for p in p64]
+    multi = HipMulti(sims)
+    tune64 = multi.autotune(2)
+    multi.synchronize()
+    t0 = time.perf_counter()
+    multi.run_async(0, STEPS)
+    multi.synchronize()
+    S64 = [s11(sim, p) for sim, p in zip(sims, p64)]
+    t64 = time.perf_counter() - t0
+    multi.close()
+    for sim in sims:
+        sim.close()
+
+    net = Predictor(weights_path)
+    sims = [HipSim(p, precision="fp32", boundary="matlab", port_dft_freqs=freqs, farfield=False) for p in p32]
+    multi = HipMulti(sims)
+    tune32 = multi.autotune(2)
+    multi.synchronize()
+    t0 = time.perf_counter()
+    multi.run_async(0, N_PREFIX)
+    feats = [features(*[a[0] for a in sim.traces(0, N_PREFIX)]) for sim in sims]
+    Y = net(np.stack([f[0] for f in feats]))
+    S, logr = [], []
+    for k, sim in enumerate(sims):
+        X = sim.port_dft()
+        tail, lr = decode(Y[k], feats[k][0])
+        Vt, It = tail_dft(tail, feats[k][1], p32[k].grid["dt"], freqs)
+        S.append(s11_from(X[0] + Vt, X[sim.n_sv] + It))
+        logr.append(lr)
+    t32 = time.perf_counter() - t0
+    multi.close()
+    for sim in sims:
+        sim.close()
+    net.close()
+    for sub in ("f64", "f32"):
+        d = os.path.join(workdir, sub)
+        for f in os.listdir(d):
+            if f.endswith(".fdtdp"):
+                os.remove(os.path.join(d, f))
+
+    fb = np.array(logr) > FALLBACK_LOG_RATIO
+    err = np.array([np.max(np.abs(20 * np.log10(np.abs(a)) - 20 * np.log10(np.abs(b)))) for a, b in zip(S, S64)])
+    err = np.where(fb, 0.0, err)
+    t_early = t32 + fb.sum() / count * t64
+    return {"variants": count, "fp64_tuning": list(tune64[:3]), "fp32_tuning": list(tune32[:3]),
+            "fp64_s_per_variant": round(t64 / count, 5), "early_stop_s_per_variant": round(t_early / count, 5),
+            "speedup": round(t64 / t_early, 2), "fallbacks": int(fb.sum()),
+            "max_err_db_max": round(float(err.max()), 4), "max_err_db_median": round(float(np.median(err)), 4),
+            "within_0p1db": round(float(np.mean(err <= 0.1)), 4)}
+
+
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("what", choices=["gen-train", "gen-heldout", "evaluate", "tts"])
+    ap.add_argument("what", choices=["gen-train", "gen-heldout", "evaluate", "tts", "tts-batched"])
     ap.add_argument("--weights", default=None)
     ap.add_argument("--heldout", default=None)
     ap.add_argument("--first", type=int, default=0)
     ap.add_argument("--count", type=int, default=100)
     ap.add_argument("--seed", type=int, default=4)
-    ap.add_argument("--inflight", type=int, default=8)
+    ap.add_argument("--batch", type=int, default=64)
     ap.add_argument("--out", default=None)
     a = ap.parse_args()
     if a.what == "evaluate":
         out = evaluate(a.weights, a.heldout)
     elif a.what == "tts":
         out = time_to_solution(a.weights, a.first, a.count, seed=a.seed)
+    elif a.what == "tts-batched":
+        out = time_to_solution_batched(a.weights, a.first, a.count, seed=a.seed)
     else:
         fn = gen_train if a.what == "gen-train" else gen_heldout
-        out = fn(a.first, a.count, a.out, seed=a.seed, inflight=a.inflight)
+        out = fn(a.first, a.count, a.out, seed=a.seed, batch=a.batch)
     text = json.dumps(out)
-    if a.out and a.what in ("evaluate", "tts"):
+    if a.out and a.what in ("evaluate", "tts", "tts-batched"):
         with open(a.out, "w") as f:
             f.write(text)
     print(text, flush=True)

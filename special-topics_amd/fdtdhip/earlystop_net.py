@@ -6,7 +6,11 @@ exactly this):
   4 x Conv1d(k=5, stride=2, pad=2) + ReLU, 64 channels: lengths 375 -> 188 -> 94 -> 47 -> 24
   flatten 1536 -> Linear 512 + ReLU -> Linear 1375
   output: 2 x 687 pooled tail samples (V then I, each in units of the prefix's final
-          amplitude: target = tail / end_amplitude) and log10 of the residual-energy ratio.
+          amplitude: target = tail / end_amplitude) and an estimate of log10 of the
+          residual energy -- the energy of the difference between the predicted and the
+          true tail -- over the run's total port energy (prefix plus tail, in the
+          prefix-normalised units of the input).  The fallback to the full-length run
+          keys on that estimate.
 Weights are BF16, activations and accumulation FP32 (training uses torch autocast in
 bfloat16 for the matrix products).
 
@@ -46,32 +50,55 @@ def make_model():
     return nn.Sequential(*layers)
 
 
+def energies(x: np.ndarray, tail: np.ndarray):
+    """(prefix energy, tail energy) in normalised units from pooled samples: each
+    input sample stands for POOL_IN steps, each tail sample for POOL_OUT."""
+    return ES.POOL_IN * np.sum(x ** 2, axis=(-2, -1)), ES.POOL_OUT * np.sum(tail ** 2, axis=(-2, -1))
+
+
 def load_data(paths):
-    xs, ys = [], []
+    """x [n,2,375], y [n,1374] (scaled tail), a [n,2] end amplitudes, e_total [n]."""
+    xs, ys, amps, etot = [], [], [], []
     for p in paths:
         d = np.load(p)
-        x, tail, lr = d["x"], d["tail"], d["log_ratio"]
+        x, tail = d["x"], d["tail"]
         a = end_amplitude(x)
-        y = np.concatenate([(tail / a[:, :, None]).reshape(len(x), -1), lr[:, None]], axis=1)
+        ep, et = energies(x, tail)
         xs.append(x)
-        ys.append(y)
-    return np.concatenate(xs).astype(np.float32), np.concatenate(ys).astype(np.float32)
+        ys.append((tail / a[:, :, None]).reshape(len(x), -1))
+        amps.append(a)
+        etot.append(ep + et)
+    return (np.concatenate(xs).astype(np.float32), np.concatenate(ys).astype(np.float32),
+            np.concatenate(amps).astype(np.float32), np.concatenate(etot).astype(np.float32))
+
+
+def residual_log_ratio(pred_tail, true_tail, a, e_total):
+    """log10 of the pooled-tail error energy over the total energy (torch tensors)."""
+    import torch
+    err = ((pred_tail - true_tail).view(-1, 2, ES.N_OUT) * a[:, :, None]) ** 2
+    return torch.log10(ES.POOL_OUT * err.sum(dim=(1, 2)) / e_total + 1e-30)
 
 
 def train(paths, out, epochs=300, batch=64, lr=2e-3, val_frac=0.1, seed=0, threads=8):
     import torch
     torch.manual_seed(seed)
     torch.set_num_threads(threads)
-    x, y = load_data(paths)
+    x, y, a, et = load_data(paths)
     rng = np.random.default_rng(seed)
     perm = rng.permutation(len(x))
     nval = max(1, int(len(x) * val_frac))
     tr, va = perm[nval:], perm[:nval]
-    X, Y = torch.from_numpy(x), torch.from_numpy(y)
-    # Tail samples and the log-ratio scalar get comparable weight.
-    wts = torch.ones(N_OUT)
-    wts[-1] = float(2 * ES.N_OUT) * 0.05
+    X, Y, A, ET = torch.from_numpy(x), torch.from_numpy(y), torch.from_numpy(a), torch.from_numpy(et)
+    n_tail = 2 * ES.N_OUT
     model = make_model()
+
+    def losses(idx, pred):
+        pt = pred[:, :n_tail].float()
+        tail_loss = ((pt - Y[idx]) ** 2).mean()
+        # The last output learns the (detached) residual energy of the current prediction.
+        target = residual_log_ratio(pt.detach(), Y[idx], A[idx], ET[idx])
+        est_loss = ((pred[:, -1].float() - target) ** 2).mean()
+        return tail_loss, est_loss
     opt = torch.optim.AdamW(model.parameters(), lr=lr, weight_decay=1e-5)
     sched = torch.optim.lr_scheduler.OneCycleLR(opt, max_lr=lr, total_steps=epochs * ((len(tr) + batch - 1) // batch))
     t0 = time.time()
@@ -83,7 +110,8 @@ def train(paths, out, epochs=300, batch=64, lr=2e-3, val_frac=0.1, seed=0, threa
             idx = torch.from_numpy(order[k:k + batch])
             with torch.autocast("cpu", dtype=torch.bfloat16):
                 pred = model(X[idx])
-            loss = (((pred.float() - Y[idx]) ** 2) * wts).mean()
+            tl, el = losses(idx, pred)
+            loss = tl + 1e-3 * el
             opt.zero_grad()
             loss.backward()
             opt.step()
@@ -92,9 +120,10 @@ def train(paths, out, epochs=300, batch=64, lr=2e-3, val_frac=0.1, seed=0, threa
             model.eval()
             with torch.no_grad(), torch.autocast("cpu", dtype=torch.bfloat16):
                 pv = model(X[va]).float()
-            vl = float((((pv - Y[va]) ** 2) * wts).mean())
-            hist.append((ep, round(float(loss), 6), round(vl, 6)))
-            print(f"[train] epoch {ep} loss {float(loss):.5f} val {vl:.5f} ({time.time() - t0:.0f} s)", flush=True)
+            vt, ve = losses(torch.from_numpy(va), pv)
+            hist.append((ep, round(tl.item(), 6), round(vt.item(), 6), round(ve.item(), 4)))
+            print(f"[train] epoch {ep} tail {tl.item():.5f} val tail {vt.item():.5f} val estimate {ve.item():.4f} "
+                  f"({time.time() - t0:.0f} s)", flush=True)
     save(model, out, {"epochs": epochs, "samples": int(len(tr)), "val": int(nval), "history": hist})
     return model
 

@@ -36,7 +36,7 @@ class FdtdHipDims(C.Structure):
 
 class FdtdHipTuning(C.Structure):
     _fields_ = [("planes_per_workgroup", C.c_int), ("cells_per_thread", C.c_int),
-                ("coefficient_storage", C.c_int)]
+                ("coefficient_storage", C.c_int), ("fuse_cpml", C.c_int)]
 
 
 class FdtdHipKernelTimes(C.Structure):
@@ -54,6 +54,8 @@ EXPORTS = [
     "fdtd_hip_set_ntff_mode", "fdtd_hip_get_ntff_mode", "fdtd_hip_set_graphs",
     "fdtd_hip_set_tuning", "fdtd_hip_get_tuning", "fdtd_hip_autotune",
     "fdtd_hip_get_coefficient_storage", "fdtd_hip_set_timing", "fdtd_hip_get_kernel_times", "fdtd_hip_synchronize", "fdtd_hip_destroy",
+    "fdtd_hip_multi_create", "fdtd_hip_multi_run_async", "fdtd_hip_multi_synchronize", "fdtd_hip_multi_set_graphs",
+    "fdtd_hip_multi_get_tuning", "fdtd_hip_multi_autotune", "fdtd_hip_multi_destroy",
     "fdtd_hip_es_create", "fdtd_hip_es_predict", "fdtd_hip_es_destroy",
 ]
 
@@ -120,6 +122,14 @@ def lib():
         L.fdtd_hip_synchronize.argtypes = [P]
         L.fdtd_hip_destroy.argtypes = [P]
         L.fdtd_hip_destroy.restype = None
+        L.fdtd_hip_multi_create.argtypes = [C.POINTER(P), I, C.POINTER(FdtdHipTuning), C.POINTER(P)]
+        L.fdtd_hip_multi_run_async.argtypes = [P, I, I]
+        L.fdtd_hip_multi_synchronize.argtypes = [P]
+        L.fdtd_hip_multi_set_graphs.argtypes = [P, I]
+        L.fdtd_hip_multi_get_tuning.argtypes = [P, C.POINTER(FdtdHipTuning)]
+        L.fdtd_hip_multi_autotune.argtypes = [P, I, C.POINTER(FdtdHipTuning)]
+        L.fdtd_hip_multi_destroy.argtypes = [P]
+        L.fdtd_hip_multi_destroy.restype = None
         L.fdtd_hip_es_create.argtypes = [C.POINTER(C.c_uint16), LL, FP, LL, C.POINTER(P)]
         L.fdtd_hip_es_predict.argtypes = [P, I, FP, FP]
         L.fdtd_hip_es_destroy.argtypes = [P]

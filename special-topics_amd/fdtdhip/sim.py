@@ -199,19 +199,25 @@ class HipSim:
         native.check(self.L.fdtd_hip_read_port_dft(self.h, _d(re), _d(im), n), "read_port_dft")
         return (re + 1j * im).reshape(self.n_sv + self.n_si, self.ndft)
 
-    def set_tuning(self, planes_per_workgroup: int = 0, cells_per_thread: int = 0, coefficient_storage: int = 0):
-        t = native.FdtdHipTuning(int(planes_per_workgroup), int(cells_per_thread), int(coefficient_storage))
+    def set_tuning(self, planes_per_workgroup: int = 0, cells_per_thread: int = 0, coefficient_storage: int = 0,
+                   fuse_cpml: int = 0):
+        t = native.FdtdHipTuning(int(planes_per_workgroup), int(cells_per_thread), int(coefficient_storage),
+                                 int(fuse_cpml))
         native.check(self.L.fdtd_hip_set_tuning(self.h, C.byref(t)), "set_tuning")
+
+    @staticmethod
+    def _tuple(t):
+        return t.planes_per_workgroup, t.cells_per_thread, t.coefficient_storage, t.fuse_cpml
 
     def tuning(self):
         t = native.FdtdHipTuning()
         native.check(self.L.fdtd_hip_get_tuning(self.h, C.byref(t)), "get_tuning")
-        return t.planes_per_workgroup, t.cells_per_thread, t.coefficient_storage
+        return self._tuple(t)
 
     def autotune(self, steps_per_candidate: int = 3):
         t = native.FdtdHipTuning()
         native.check(self.L.fdtd_hip_autotune(self.h, int(steps_per_candidate), C.byref(t)), "autotune")
-        return t.planes_per_workgroup, t.cells_per_thread, t.coefficient_storage
+        return self._tuple(t)
 
     def coefficient_storage(self):
         mode = C.c_int(0)
@@ -240,3 +246,52 @@ class HipSim:
         t = native.FdtdHipKernelTimes()
         native.check(self.L.fdtd_hip_get_kernel_times(self.h, C.byref(t)), "get_kernel_times")
         return list(t.ms), list(t.launches)
+
+
+class HipMulti:
+    """Several HipSim contexts stepped together, one launch per phase over all of them
+    (fdtd_hip_multi_*).  The contexts stay usable for reads (traces, port_dft,
+    farfield, field, ...) after run(); they must not be run on their own meanwhile.
+    Each context's results equal its own run with the same tuning, bit for bit."""
+
+    def __init__(self, sims, planes_per_workgroup: int = 0, cells_per_thread: int = 0,
+                 coefficient_storage: int = 0):
+        self.sims = list(sims)
+        self.L = native.lib()
+        self.h = C.c_void_p()
+        arr = (C.c_void_p * len(self.sims))(*[s.h for s in self.sims])
+        t = native.FdtdHipTuning(int(planes_per_workgroup), int(cells_per_thread), int(coefficient_storage), 2)
+        native.check(self.L.fdtd_hip_multi_create(arr, len(self.sims), C.byref(t), C.byref(self.h)), "multi_create")
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.fdtd_hip_multi_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def run_async(self, ts0: int, count: int):
+        native.check(self.L.fdtd_hip_multi_run_async(self.h, ts0, count), "multi_run_async")
+
+    def synchronize(self):
+        native.check(self.L.fdtd_hip_multi_synchronize(self.h), "multi_synchronize")
+
+    def run(self, ts0: int, count: int):
+        """Steps [ts0, ts0+count) of every context; returns [(V, I)] per context."""
+        self.run_async(ts0, count)
+        return [s.traces(ts0, count) for s in self.sims]
+
+    def set_graphs(self, enable: bool):
+        native.check(self.L.fdtd_hip_multi_set_graphs(self.h, 1 if enable else 0), "multi_set_graphs")
+
+    def tuning(self):
+        t = native.FdtdHipTuning()
+        native.check(self.L.fdtd_hip_multi_get_tuning(self.h, C.byref(t)), "multi_get_tuning")
+        return HipSim._tuple(t)
+
+    def autotune(self, steps_per_candidate: int = 3):
+        t = native.FdtdHipTuning()
+        native.check(self.L.fdtd_hip_multi_autotune(self.h, int(steps_per_candidate), C.byref(t)), "multi_autotune")
+        return HipSim._tuple(t)
+

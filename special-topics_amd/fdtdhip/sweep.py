@@ -1,14 +1,19 @@
 """Batched sweep of many problems on one GPU.
 
-Each problem gets its own context (device state and HIP stream); `inflight` of them
-run at once, enqueued with fdtd_hip_run_batch_async, and the host prepares the next
-problem (builder process, upload, coefficient deduplication) while the GPU runs the
-others.  Per problem the sweep keeps the port traces, S11 from the on-GPU running port
+Each problem gets its own context (device state and HIP stream).  Two ways to run
+them:
+  * batched (``batch=N``): N contexts at a time are stepped together by one launch per
+    phase (fdtd_hip_multi_*, tuned once by its autotuner), while the host uploads the
+    next N;
+  * in flight (``batch=0``): ``inflight`` contexts run at once, each enqueued on its
+    own stream with fdtd_hip_run_batch_async.
+Either way the host prepares the next problems (builder process, upload, coefficient
+deduplication) while the GPU runs.  Per problem the sweep keeps the port traces, S11 from the on-GPU running port
 DFT (sparameters.js:23-101 from dft.js:17-38 sums), the resonance (S11 minimum) and the
 directivity cuts of js_fdtd/src/index.js:77-92 at the NTFF frequency nearest it
 (farfield.js:257-607, evaluated on the GPU).
 
-    python -m fdtdhip.sweep SCENARIO.json ... [--inflight 4] [--precision fp32]
+    python -m fdtdhip.sweep SCENARIO.json ... [--batch 64 | --batch 0 --inflight 4] [--precision fp32]
 """
 
 from __future__ import annotations
@@ -23,7 +28,7 @@ import time
 import numpy as np
 
 from .problem import build_problem, read_problem
-from .sim import HipSim
+from .sim import HipMulti, HipSim
 
 
 def standard_cuts():
@@ -87,24 +92,29 @@ class LazyBuilds:
 
 
 class Sweep:
-    """Runs problems with up to `inflight` in flight; results in submission order."""
+    """Runs problems `batch` at a time in one batched launch (batch > 0), or with up to
+    `inflight` single-problem contexts in flight (batch = 0); results in submission
+    order."""
 
     def __init__(self, precision="fp32", boundary="matlab", inflight=4, tuning=None, keep_traces=True,
-                 farfield=True, steps=None, s11=True, progress_every=0):
+                 farfield=True, steps=None, s11=True, progress_every=0, batch=0):
         self.precision, self.boundary, self.inflight = precision, boundary, inflight
+        self.batch = batch
+        self.batch_tuning = None
         self.tuning, self.keep_traces, self.farfield, self.steps = tuning, keep_traces, farfield, steps
         self.with_s11 = s11
         self.progress_every = progress_every
         self.cuts = standard_cuts()
 
-    def _start(self, idx, problem):
+    def _start(self, idx, problem, launch=True):
         freqs = frequency_grid(problem.header.get("dft"))
         sim = HipSim(problem, precision=self.precision, boundary=self.boundary,
                      port_dft_freqs=freqs if self.with_s11 else None, farfield=self.farfield)
-        if self.tuning is not None:
+        if self.tuning is not None and launch:
             sim.set_tuning(*self.tuning)
         steps = self.steps or problem.steps
-        sim.run_async(0, steps)
+        if launch:
+            sim.run_async(0, steps)
         return {"idx": idx, "sim": sim, "problem": problem, "freqs": freqs, "steps": steps}
 
     def _finish(self, job):
@@ -135,11 +145,63 @@ class Sweep:
         sim.close()
         return res
 
+    def _load(self, items, lazy, k):
+        p = items.get(k) if lazy else items[k]
+        path = p if isinstance(p, str) else None
+        return read_problem(p) if path else p, path
+
+    def _launch_group(self, jobs):
+        """One batched launch over the group's contexts (tuned on the first group)."""
+        steps = {j["steps"] for j in jobs}
+        if len(steps) != 1:
+            raise ValueError("a batched group needs one step count")
+        if self.batch_tuning is None:
+            if self.tuning is not None:
+                self.batch_tuning = tuple(self.tuning[:3])
+            else:
+                m = HipMulti([j["sim"] for j in jobs])
+                self.batch_tuning = m.autotune(2)[:3]
+                m.close()
+        multi = HipMulti([j["sim"] for j in jobs], *self.batch_tuning)
+        multi.run_async(0, steps.pop())
+        return multi
+
+    def _run_batched(self, items, lazy, t0):
+        results = [None] * len(items)
+        cell_steps = 0
+
+        def group(first):
+            jobs = []
+            for k in range(first, min(first + self.batch, len(items))):
+                p, path = self._load(items, lazy, k)
+                jobs.append(self._start(k, p, launch=False))
+                if path and lazy:
+                    os.remove(path)
+            return jobs
+
+        nxt = 0
+        jobs = group(nxt)
+        while jobs:
+            nxt += len(jobs)
+            multi = self._launch_group(jobs)
+            for j in jobs:
+                cell_steps += int(np.prod(j["problem"].dims)) * j["steps"]
+            following = group(nxt)            # uploads overlap the running group
+            for j in jobs:
+                results[j["idx"]] = self._finish(j)
+            multi.close()
+            if self.progress_every:
+                print(f"[sweep] {nxt}/{len(items)} done, {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+            jobs = following
+        return results, time.perf_counter() - t0, cell_steps
+
     def run(self, problems):
         """problems: Problems, problem-file paths, or a LazyBuilds (files deleted after
         upload).  Returns (results, seconds, cell_steps)."""
         lazy = isinstance(problems, LazyBuilds)
         items = problems if lazy else list(problems)
+        if self.batch > 0:
+            return self._run_batched(items, lazy, time.perf_counter())
         results = [None] * len(items)
         running = []
         t0 = time.perf_counter()
@@ -173,10 +235,11 @@ def main():
     ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64"])
     ap.add_argument("--boundary", default="matlab", choices=["matlab", "reference"])
     ap.add_argument("--inflight", type=int, default=4)
+    ap.add_argument("--batch", type=int, default=64, help="variants per batched launch (0: streams in flight)")
     ap.add_argument("--out", default=None)
     a = ap.parse_args()
     files = build_all(a.scenarios, a.workdir, fp32=a.precision == "fp32")
-    res, secs, cells = Sweep(a.precision, a.boundary, a.inflight, keep_traces=False).run(files)
+    res, secs, cells = Sweep(a.precision, a.boundary, a.inflight, keep_traces=False, batch=a.batch).run(files)
     summary = {"variants": len(res), "seconds": round(secs, 3), "mcell_updates_per_s": round(cells / secs / 1e6, 1),
                "results": [{k: v for k, v in r.items() if k not in ("s11", "s11_db", "cuts", "v", "i")} for r in res]}
     text = json.dumps(summary)

@@ -77,8 +77,8 @@ export function resolveOptions(options) {
   if (!(boundary in BOUNDARY)) throw new Error(`hip backend: unknown boundary "${boundary}" (reference | matlab)`);
   let tuning = pick(o.tuning, env.FDTD_HIP_TUNING || 'auto');
   if (typeof tuning === 'string' && tuning !== 'auto' && tuning !== 'default') {
-    const [p, c, s] = tuning.split(',').map(Number);
-    tuning = { planesPerWorkgroup: p || 0, cellsPerThread: c || 0, coefficientStorage: s || 0 };
+    const [p, c, s, z] = tuning.split(',').map(Number);
+    tuning = { planesPerWorkgroup: p || 0, cellsPerThread: c || 0, coefficientStorage: s || 0, fuseCpml: z || 0 };
   }
   const batchSize = pick(o.batchSize, 50);
   if (!(Number.isInteger(batchSize) && batchSize > 0)) throw new Error(`hip backend: bad batchSize ${batchSize}`);
@@ -178,6 +178,52 @@ function farfieldOnDevice(A, h, ffHeader, cuts) {
   };
 }
 
+/**
+ * After the steps have run: the NTFF accumulators (and, with farfieldCuts, the GPU
+ * far-field post), the running port DFTs and, with readFields, the fields.
+ */
+function collectResults(A, h, problem, opts, header, ff) {
+  const { grid, samplers } = problem;
+  const sampledVoltages = (samplers && samplers.sampledVoltages) || [];
+  const sampledCurrents = (samplers && samplers.sampledCurrents) || [];
+  const nsv = sampledVoltages.length, nsi = sampledCurrents.length;
+  if (header.ff.nFreq > 0) {
+    FF_ORDER.forEach((name, q) => A.readFarfield(h, q, ff[name].re, ff[name].im));
+    if (opts.farfieldCuts) problem.farfieldPost = farfieldOnDevice(A, h, header.ff, opts.farfieldCuts);
+  }
+  if (opts.portDftFrequencies && opts.portDftFrequencies.length > 0) {
+    const m = opts.portDftFrequencies.length;
+    const re = new Float64Array((nsv + nsi) * m), im = new Float64Array((nsv + nsi) * m);
+    A.readPortDft(h, re, im);
+    const part = (o) => ({ re: re.slice(o * m, (o + 1) * m), im: im.slice(o * m, (o + 1) * m) });
+    problem.portDft = {
+      frequencies: opts.portDftFrequencies,
+      voltages: sampledVoltages.map((_, o) => part(o)),
+      currents: sampledCurrents.map((_, o) => part(nsv + o)),
+    };
+  }
+  if (opts.readFields && grid.fields) {
+    FIELD_ORDER.forEach((name, q) => { if (grid.fields[name]) A.readField(h, q, grid.fields[name]); });
+  }
+}
+
+function checkProbes(problem) {
+  const { samplers } = problem;
+  const probes = ((samplers && samplers.sampledEFields) || []).length + ((samplers && samplers.sampledHFields) || []).length;
+  if (probes > 0) {
+    throw new Error('hip backend: point field probes (sampledEFields / sampledHFields) are not supported');
+  }
+}
+
+function requireAddon() {
+  const A = loadAddon();
+  if (!A) {
+    const why = addonError ? addonError.message : 'not found';
+    throw new Error(`hip addon ${ADDON_PATH} could not be loaded (${why}); build it with __graft_entry__.build()`);
+  }
+  return A;
+}
+
 let lastRun = null;
 
 /** @type {import('./types.js').Backend} */
@@ -200,20 +246,13 @@ const backend = {
   },
 
   async *run(problem) {
-    const A = loadAddon();
-    if (!A) {
-      const why = addonError ? addonError.message : 'not found';
-      throw new Error(`hip addon ${ADDON_PATH} could not be loaded (${why}); build it with __graft_entry__.build()`);
-    }
+    const A = requireAddon();
     const opts = resolveOptions(problem.options);
     const { grid, samplers } = problem;
     const total = grid.numberOfTimeSteps;
     const sampledVoltages = (samplers && samplers.sampledVoltages) || [];
     const sampledCurrents = (samplers && samplers.sampledCurrents) || [];
-    const probes = ((samplers && samplers.sampledEFields) || []).length + ((samplers && samplers.sampledHFields) || []).length;
-    if (probes > 0) {
-      throw new Error('hip backend: point field probes (sampledEFields / sampledHFields) are not supported');
-    }
+    checkProbes(problem);
     const tSetup = Date.now();
     const { h, header, ff } = uploadProblem(A, problem, opts);
     try {
@@ -235,27 +274,57 @@ const backend = {
         yield snapshot(start + count - 1, total, startTime, sampledVoltages, grid.dt);
         await new Promise((r) => setTimeout(r, 0));
       }
-      if (header.ff.nFreq > 0) {
-        FF_ORDER.forEach((name, q) => A.readFarfield(h, q, ff[name].re, ff[name].im));
-        if (opts.farfieldCuts) problem.farfieldPost = farfieldOnDevice(A, h, header.ff, opts.farfieldCuts);
-      }
-      if (opts.portDftFrequencies && opts.portDftFrequencies.length > 0) {
-        const m = opts.portDftFrequencies.length;
-        const re = new Float64Array((nsv + nsi) * m), im = new Float64Array((nsv + nsi) * m);
-        A.readPortDft(h, re, im);
-        const part = (o) => ({ re: re.slice(o * m, (o + 1) * m), im: im.slice(o * m, (o + 1) * m) });
-        problem.portDft = {
-          frequencies: opts.portDftFrequencies,
-          voltages: sampledVoltages.map((_, o) => part(o)),
-          currents: sampledCurrents.map((_, o) => part(nsv + o)),
-        };
-      }
-      if (opts.readFields && grid.fields) {
-        FIELD_ORDER.forEach((name, q) => { if (grid.fields[name]) A.readField(h, q, grid.fields[name]); });
-      }
+      collectResults(A, h, problem, opts, header, ff);
       lastRun.run_s = (Date.now() - startTime) / 1000;
     } finally {
       A.destroy(h);
+    }
+  },
+
+  /**
+   * Many problems (of any grid size) run together: one launch per phase covers all of
+   * them (fdtd_hip_multi_*).  Every problem gets the same results as from run(); the
+   * precision, step count and tuning come from the first problem's options, which
+   * must agree on precision and step count.  Resolves once all have finished.
+   */
+  async runMany(problems) {
+    const A = requireAddon();
+    if (!Array.isArray(problems) || problems.length === 0) throw new Error('hip backend: runMany needs problems');
+    const opts = problems.map((p) => resolveOptions(p.options));
+    const total = problems[0].grid.numberOfTimeSteps;
+    problems.forEach((p, n) => {
+      checkProbes(p);
+      if (opts[n].precision !== opts[0].precision) throw new Error('hip backend: runMany needs one precision');
+      if (p.grid.numberOfTimeSteps !== total) throw new Error('hip backend: runMany needs one step count');
+    });
+    const tSetup = Date.now();
+    const uploaded = [];
+    let m = null;
+    try {
+      for (let n = 0; n < problems.length; n++) uploaded.push(uploadProblem(A, problems[n], opts[n]));
+      const t = opts[0].tuning;
+      m = A.multiCreate(uploaded.map((u) => u.h), t && typeof t === 'object' ? t : null);
+      const tuning = t === 'auto' ? A.multiAutotune(m, opts[0].autotuneSteps) : null;
+      lastRun = { precision: opts[0].precision, problems: problems.length, tuning,
+                  setup_s: (Date.now() - tSetup) / 1000 };
+      const startTime = Date.now();
+      A.multiRun(m, 0, total);
+      A.multiSynchronize(m);
+      problems.forEach((p, n) => {
+        const { h, header, ff } = uploaded[n];
+        const sv = (p.samplers && p.samplers.sampledVoltages) || [];
+        const si = (p.samplers && p.samplers.sampledCurrents) || [];
+        const v = new Float64Array(Math.max(sv.length, 1) * total);
+        const c = new Float64Array(Math.max(si.length, 1) * total);
+        A.readTraces(h, 0, total, sv.length ? v : null, si.length ? c : null);
+        sv.forEach((o, k) => o.sampled_value.set(v.subarray(k * total, (k + 1) * total)));
+        si.forEach((o, k) => o.sampled_value.set(c.subarray(k * total, (k + 1) * total)));
+        collectResults(A, h, p, opts[n], header, ff);
+      });
+      lastRun.run_s = (Date.now() - startTime) / 1000;
+    } finally {
+      if (m) A.multiDestroy(m);
+      for (const u of uploaded) A.destroy(u.h);
     }
   },
 };

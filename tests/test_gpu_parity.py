@@ -41,15 +41,17 @@ def _check_run_golden(sim, fx, fields=True):
     return v, i
 
 
+@pytest.mark.parametrize("fuse", [0, 1, 2])
 @pytest.mark.parametrize("storage,expect", [(0, 1), (1, 0)])
 @pytest.mark.parametrize("cells", [1, 2])
-def test_golden_small_fp64_matches_reference_run(get_problem, storage, expect, cells):
-    """Every coefficient storage (deduplicated indices, per-cell arrays) and launch
-    shape reproduces the reference run bit for bit."""
+def test_golden_small_fp64_matches_reference_run(get_problem, storage, expect, cells, fuse):
+    """Every coefficient storage (deduplicated indices, per-cell arrays), launch shape
+    and CPML placement (separate pass, z slabs fused into the bulk kernels, every face
+    fused) reproduces the reference run bit for bit."""
     prob = get_problem("golden-small")
     fx = G.load("run-golden-small.json")
     sim = _sim(prob)
-    sim.set_tuning(0, cells, storage)
+    sim.set_tuning(0, cells, storage, fuse)
     mode, entries = sim.coefficient_storage()
     assert mode == expect and (expect == 0 or 0 < max(entries) <= 256)
     _check_run_golden(sim, fx)
@@ -66,16 +68,18 @@ def test_march_depth_does_not_change_results(get_problem, planes, cells):
     sim.close()
 
 
+@pytest.mark.parametrize("fuse", [0, 1, 2])
 @pytest.mark.parametrize("cells", [1, 2, 4])
-def test_fp32_launch_shapes_agree(get_problem, cells):
-    """FP32 results do not depend on the launch shape either (same arithmetic)."""
+def test_fp32_launch_shapes_agree(get_problem, cells, fuse):
+    """FP32 results do not depend on the launch shape or on where the z-slab CPML
+    terms are applied (same arithmetic, same order per element)."""
     prob = get_problem("golden-small")
     ref = _sim(prob, precision="fp32")
     ref.set_tuning(1, 1, 1)
     v0, i0 = ref.run(0, 100)
     ref.close()
     sim = _sim(prob, precision="fp32")
-    sim.set_tuning(5, cells, 0)
+    sim.set_tuning(5, cells, 0, fuse)
     v1, i1 = sim.run(0, 100)
     sim.close()
     assert np.array_equal(v0, v1) and np.array_equal(i0, i1)
@@ -86,8 +90,8 @@ def test_autotune_then_run_matches_reference(get_problem):
     prob = get_problem("golden-small")
     fx = G.load("run-golden-small.json")
     sim = _sim(prob)
-    planes, cells, storage = sim.autotune(2)
-    assert planes >= 1 and cells in (1, 2) and storage in (0, 1)
+    planes, cells, storage, fuse = sim.autotune(2)
+    assert planes >= 1 and cells in (1, 2) and storage in (0, 1) and fuse in (0, 1, 2)
     _check_run_golden(sim, fx)
     sim.close()
 
@@ -115,9 +119,11 @@ def test_random_state_one_step_fp64_matches_oracle(get_problem):
     sim.close()
 
 
-def test_matlab_boundary_fp64_matches_oracle(get_problem):
+@pytest.mark.parametrize("fuse", [0, 1, 2])
+def test_matlab_boundary_fp64_matches_oracle(get_problem, fuse):
     prob = get_problem("golden-small")
     sim = _sim(prob, boundary="matlab")
+    sim.set_tuning(0, 0, 0, fuse)
     ref = OracleSim(prob, boundary="matlab")
     v, i = sim.run(0, 120)
     rv, ri = ref.run(0, 120)
@@ -140,12 +146,15 @@ def test_batches_compose(get_problem):
     assert np.array_equal(ia, np.concatenate([p[1] for p in parts], axis=1))
 
 
-@pytest.mark.parametrize("scenario", ["ifa-matlab-7000", "ifa-config1-7000"])
-def test_ifa_fp64_full_run_matches_reference(get_problem, scenario):
+@pytest.mark.parametrize("scenario,fuse", [("ifa-matlab-7000", 0), ("ifa-matlab-7000", 2),
+                                           ("ifa-config1-7000", 0), ("ifa-config1-7000", 1),
+                                           ("ifa-config1-7000", 2)])
+def test_ifa_fp64_full_run_matches_reference(get_problem, scenario, fuse):
     """The 39x136x136 IFA, 7000 FP64 steps, against the reference's own per-step code."""
     prob = get_problem(scenario)
     fx = G.load(f"run-{scenario}.json")
     sim = _sim(prob)
+    sim.set_tuning(0, 0, 0, fuse)
     _check_run_golden(sim, fx)
     sim.close()
 

@@ -118,7 +118,8 @@ def test_backend_option_validation(tmp_path):
     out = json.loads(res.stdout)
     assert out["dflt"]["precision"] == "fp64" and out["dflt"]["boundary"] == "reference"
     assert out["dflt"]["tuning"] == "auto" and out["dflt"]["batchSize"] == 50
-    assert out["tuned"]["tuning"] == {"planesPerWorkgroup": 2, "cellsPerThread": 4, "coefficientStorage": 1}
+    assert out["tuned"]["tuning"] == {"planesPerWorkgroup": 2, "cellsPerThread": 4, "coefficientStorage": 1,
+                                      "fuseCpml": 0}
     assert out["tuned"]["precision"] == "fp32" and out["tuned"]["batchSize"] == 7
     assert all(out[k] == "rejected" for k in out if k.startswith("{"))
 
@@ -204,3 +205,33 @@ def test_cli_writes_reference_schema(tmp_path):
         # log10 of V8 and of numpy may differ in the last bit.
         assert np.allclose(np.array(xy[0]["dataTheta_dB"]), 10 * np.log10(np.maximum(ref, 1e-10)),
                            rtol=1e-14, atol=1e-12)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_hip_backend_run_many_matches_run(tmp_path, precision):
+    """backend.runMany (one batched launch per phase over every problem) gives each
+    problem exactly what backend.run gives it alone."""
+    import sys
+    sys.path.insert(0, os.path.join(REPO, "special-topics_amd", "scenarios"))
+    import gen_ifa_variants as V
+    variant = tmp_path / "variant.json"
+    variant.write_text(json.dumps(V.scenario(4, 7, steps=200)))
+    scen = [SCEN["golden-small"], SCEN["ifa-matlab-7000"], str(variant)]
+    # One 200-step batch on both sides: the FP32 path folds its NTFF samples on the
+    # matrix cores per block, and a batch end closes a block, so equal batching gives
+    # equal sums.
+    job = {"scenarios": scen, "steps": 200, "out": str(tmp_path / "many.json"),
+           "options": {"precision": precision, "boundary": "matlab", "tuning": "0,1,0,2", "batchSize": 200,
+                       "portDftFrequencies": [1e9, 2.5e9, 4e9]}}
+    jp = tmp_path / "job.json"
+    jp.write_text(json.dumps(job))
+    print(f"[node backend] runMany {precision} over {len(scen)} scenarios", flush=True)
+    res = node([os.path.join(REPO, "tests", "js", "many_driver.mjs"), str(jp)])
+    assert res.returncode == 0, res.stderr[-3000:]
+    rec = json.loads((tmp_path / "many.json").read_text())
+    assert len({tuple(r["dims"]) for r in rec["single"]}) == 3
+    for n, (a, b) in enumerate(zip(rec["single"], rec["many"])):
+        diff = [k for k in a if a[k] != b.get(k)]
+        diff += [f"{k}.{q}" for k in ("fields", "ff") for q in a[k] if a[k][q] != b[k].get(q)]
+        assert a == b, f"scenario {scen[n]}: runMany differs from run in {diff}"
