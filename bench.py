@@ -32,11 +32,14 @@ FP32_BYTES_PER_CELL_PHASE = 72  # 3 own-field reads + 3 writes + 3 neighbour-fie
 PSI_BYTES = 12                  # read Psi, write Psi, read CPsi (FP32)
 
 
-def psi_counts(problem):
-    """(H-phase Psi elements, E-phase Psi elements) of the problem's CPML faces."""
+def psi_counts(problem, axes="xyz"):
+    """(H-phase Psi elements, E-phase Psi elements) of the problem's CPML faces along
+    `axes`."""
     slot_comp = {"x": ["Hy", "Hz", "Ey", "Ez"], "y": ["Hx", "Hz", "Ez", "Ex"], "z": ["Hx", "Hy", "Ex", "Ey"]}
     nh = ne = 0
     for f in problem.header["cpml"]:
+        if f["face"][0] not in axes:
+            continue
         axis = "xyz".index(f["face"][0])
         for slot in range(4):
             d = list(problem.component_dims(slot_comp[f["face"][0]][slot]))
@@ -80,6 +83,43 @@ def measured_traffic(workload: str, precision: str, kernel: str):
     except (OSError, ValueError, KeyError):
         return None
     return int(k["bytes"]) if k else None
+
+
+def pmc_summary(fetch_dir: str, write_dir: str, copy_mib: int, out: str):
+    """Per-kernel HBM bytes per dispatch from two rocprofv3 counter runs of the same
+    command (`--pmc FETCH_SIZE`, then `--pmc WRITE_SIZE`, each with
+    --copy-calibration-mib), each counter scaled so that the calibration copy
+    (__amd_rocclr_copyBuffer of copy_mib MiB read and written) reads its true size."""
+    import csv
+    import glob
+
+    def per_kernel(d, counter):
+        rows = []
+        for f in glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
+            with open(f) as fh:
+                rows += [r for r in csv.DictReader(fh) if r["Counter_Name"] == counter]
+        acc = {}
+        for r in rows:
+            acc.setdefault(r["Kernel_Name"], []).append(float(r["Counter_Value"]))
+        # The calibration copy is the largest copy dispatch (uploads are copies too).
+        return {k: (max(v) if k == "__amd_rocclr_copyBuffer" else sum(v) / len(v)) for k, v in acc.items()}
+
+    fetch, write = per_kernel(fetch_dir, "FETCH_SIZE"), per_kernel(write_dir, "WRITE_SIZE")
+    copy_kib = copy_mib * 1024
+    fs = copy_kib / fetch["__amd_rocclr_copyBuffer"]
+    ws = copy_kib / write["__amd_rocclr_copyBuffer"]
+    kernels = {}
+    for k in sorted(set(fetch) & set(write)):
+        if k.startswith("fdtd::") or k.startswith("void fdtd::"):
+            name = k[len("void "):] if k.startswith("void ") else k
+            name = name.split("(")[0]
+            kernels[name] = {"fetch_kib": fetch[k], "write_kib": write[k],
+                             "bytes": (fetch[k] * fs + write[k] * ws) * 1024}
+    doc = {"calibration": {"copy_bytes": copy_mib << 20, "fetch_scale": fs, "write_scale": ws}, "kernels": kernels}
+    with open(out, "w") as f:
+        json.dump(doc, f, indent=1)
+    for k, v in kernels.items():
+        print(f"{k}: {v['bytes'] / 1e6:.1f} MB/dispatch")
 
 
 def standard_cuts():
@@ -253,7 +293,13 @@ def main():
     ap.add_argument("--coef", type=int, default=0, help="coefficient storage (with --planes)")
     ap.add_argument("--fuse-cpml", type=int, default=0,
                     help="CPML placement with --planes: 0 separate pass, 1 z slabs fused, 2 all faces fused")
+    ap.add_argument("--pmc-summary", nargs=3, metavar=("FETCH_DIR", "WRITE_DIR", "OUT_JSON"),
+                    help="write the calibrated per-kernel traffic summary of two rocprofv3 --pmc runs and exit")
     args = ap.parse_args()
+
+    if args.pmc_summary:
+        pmc_summary(args.pmc_summary[0], args.pmc_summary[1], args.copy_calibration_mib or 1024, args.pmc_summary[2])
+        return
 
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
@@ -322,12 +368,14 @@ def main():
     bytes_h = bulk_bytes + PSI_BYTES * nh * esz // 4
     bytes_e = bulk_bytes + PSI_BYTES * ne * esz // 4
     dom = 0 if avg_ms[0] >= avg_ms[1] else 1
-    dom_bytes = bulk_bytes
+    # A stencil kernel that applies CPML terms itself also moves those faces' Psi bytes.
+    fused_axes = {0: "", 1: "z", 2: "xyz"}[tuning[3]]
+    fh, fe = psi_counts(problem, fused_axes) if fused_axes else (0, 0)
+    dom_bytes = bulk_bytes + PSI_BYTES * (fh if dom == 0 else fe) * esz // 4
     achieved = dom_bytes / (avg_ms[dom] * 1e-3) / 1e9
 
     ctype = "float" if fp32 else "double"
-    kname = (f"fdtd::k_update_{'h' if dom == 0 else 'e'}<{ctype}, {coef_mode}, {max(tuning[1], 1)}, "
-             f"{'true' if tuning[3] else 'false'}>")
+    kname = f"fdtd::k_update_{'h' if dom == 0 else 'e'}<{ctype}, {coef_mode}, {max(tuning[1], 1)}, {tuning[3]}>"
     traffic = measured_traffic(args.workload, args.precision, kname)
 
     mcells = ncells * args.steps / elapsed / 1e6

@@ -232,20 +232,19 @@ int fdtd_hip_synchronize(FdtdHipContext *ctx);
 void fdtd_hip_destroy(FdtdHipContext *ctx);
 
 /* Many problems ("variants", of any grid size) stepped together: one launch per
- * phase covers every variant (the H and E stencils with all CPML faces fused, the port
- * epilogue, the NTFF surface work), and whole 64-step chunks replay a captured HIP
- * graph.  Each variant's results are bit-identical to running its context alone with
+ * phase covers every variant (the H and E stencils, the CPML pass, the port epilogue,
+ * the NTFF surface work), and whole 64-step chunks replay a captured HIP graph.  Each variant's results are bit-identical to running its context alone with
  * the same tuning, and are read with the per-context calls above (read_traces,
  * read_port_dft, read_farfield, ...) once the run has finished.  No counterpart in
  * the reference, which runs one problem per process (the MATLAB sweep scripts loop
  * over scenarios).
  *
  * The contexts are borrowed, not owned: they must share a device and precision and
- * must not be run on their own while a batched run is in flight.  Tuning: planes per
- * workgroup (0 = each variant's default), cells per thread and coefficient storage as
- * in FdtdHipTuning; fuse_cpml must be 0 or 2 (every face is fused).  When the
- * variants' deduplicated coefficient tables differ in index width, per-cell arrays are
- * used for all. */
+ * must not be run on their own while a batched run is in flight.  Tuning as in
+ * FdtdHipTuning, one choice for the whole batch (planes per workgroup 0 = each
+ * variant's default); with fuse_cpml 0 or 1 the CPML cells left to the separate pass
+ * are covered by one more batched launch per phase.  When the variants' deduplicated
+ * coefficient tables differ in index width, per-cell arrays are used for all. */
 typedef struct FdtdHipMulti FdtdHipMulti;
 int fdtd_hip_multi_create(FdtdHipContext *const *ctxs, int n, const FdtdHipTuning *tuning, FdtdHipMulti **out);
 /* Steps [tsStart, tsStart+count) of every variant, enqueued; each context's own stream
@@ -255,8 +254,9 @@ int fdtd_hip_multi_synchronize(FdtdHipMulti *m);
 /* As fdtd_hip_set_graphs, for the batch (default on). */
 int fdtd_hip_multi_set_graphs(FdtdHipMulti *m, int enable);
 int fdtd_hip_multi_get_tuning(FdtdHipMulti *m, FdtdHipTuning *tuning);
-/* Time the batched launches for every cells-per-thread / planes / storage choice over a
- * few steps, keep the fastest, and reset every variant (before its first step only). */
+/* Time the batched launches for every cells-per-thread / planes / storage / CPML
+ * placement choice over a few steps, keep the fastest, and reset every variant (before
+ * its first step only). */
 int fdtd_hip_multi_autotune(FdtdHipMulti *m, int steps_per_candidate, FdtdHipTuning *chosen);
 void fdtd_hip_multi_destroy(FdtdHipMulti *m);
 

@@ -13,6 +13,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -375,24 +376,30 @@ MarchGeom march_geom(const FdtdHipContext *c, int planes) {
 
 int fuse_cpml(const FdtdHipContext *c) { return c->tune.fuse_cpml; }
 
+// Four cells per thread exist for FP32 only (check_tuning refuses them for FP64).
 template <typename T, int CM, int FC>
 void launch_h_w(FdtdHipContext *c, const Grid<T> &g, const MarchGeom &m, const FaceSet<T> &fs) {
   const dim3 grid(8 * m.per_xcd), block(256);
-  switch (cells_per_thread(c)) {
-    case 4: hipLaunchKernelGGL((k_update_h<T, CM, 4, FC>), grid, block, 0, c->stream, g, m, fs); break;
-    case 2: hipLaunchKernelGGL((k_update_h<T, CM, 2, FC>), grid, block, 0, c->stream, g, m, fs); break;
-    default: hipLaunchKernelGGL((k_update_h<T, CM, 1, FC>), grid, block, 0, c->stream, g, m, fs); break;
+  const int w = cells_per_thread(c);
+  if constexpr (sizeof(T) == 4) {
+    if (w == 4) { hipLaunchKernelGGL((k_update_h<T, CM, 4, FC>), grid, block, 0, c->stream, g, m, fs); return; }
   }
+  if (w == 2) hipLaunchKernelGGL((k_update_h<T, CM, 2, FC>), grid, block, 0, c->stream, g, m, fs);
+  else hipLaunchKernelGGL((k_update_h<T, CM, 1, FC>), grid, block, 0, c->stream, g, m, fs);
 }
 
 template <typename T, int CM, int FC>
 void launch_e_w(FdtdHipContext *c, const Grid<T> &g, const MarchGeom &m, int ghost, const FaceSet<T> &fs) {
   const dim3 grid(8 * m.per_xcd), block(256);
-  switch (cells_per_thread(c)) {
-    case 4: hipLaunchKernelGGL((k_update_e<T, CM, 4, FC>), grid, block, 0, c->stream, g, m, ghost, fs); break;
-    case 2: hipLaunchKernelGGL((k_update_e<T, CM, 2, FC>), grid, block, 0, c->stream, g, m, ghost, fs); break;
-    default: hipLaunchKernelGGL((k_update_e<T, CM, 1, FC>), grid, block, 0, c->stream, g, m, ghost, fs); break;
+  const int w = cells_per_thread(c);
+  if constexpr (sizeof(T) == 4) {
+    if (w == 4) {
+      hipLaunchKernelGGL((k_update_e<T, CM, 4, FC>), grid, block, 0, c->stream, g, m, ghost, fs);
+      return;
+    }
   }
+  if (w == 2) hipLaunchKernelGGL((k_update_e<T, CM, 2, FC>), grid, block, 0, c->stream, g, m, ghost, fs);
+  else hipLaunchKernelGGL((k_update_e<T, CM, 1, FC>), grid, block, 0, c->stream, g, m, ghost, fs);
 }
 
 template <typename T, int FC>
@@ -452,10 +459,13 @@ CpmlBoxes cpml_boxes(const FdtdHipContext *c, int phase) {
   CpmlBoxes bx{};
   auto add = [&](std::pair<int, int> x, std::pair<int, int> y, std::pair<int, int> z) {
     if (x.second <= x.first || y.second <= y.first || z.second <= z.first || bx.nbox >= MAX_CPML_BOXES) return;
+    const long long end = bx.start[bx.nbox] + (long long)(x.second - x.first) * (y.second - y.first) *
+                                                 (z.second - z.first);
+    if (end > INT_MAX) { bx.overflow = 1; return; }
     const int b = bx.nbox++;
     bx.lo[b][0] = x.first; bx.lo[b][1] = y.first; bx.lo[b][2] = z.first;
     bx.ext[b][0] = x.second - x.first; bx.ext[b][1] = y.second - y.first; bx.ext[b][2] = z.second - z.first;
-    bx.start[b + 1] = bx.start[b] + (long long)bx.ext[b][0] * bx.ext[b][1] * bx.ext[b][2];
+    bx.start[b + 1] = (int)end;
   };
   const std::pair<int, int> fy{0, n[1] + 1}, fz{0, n[2] + 1};
   if (fuse_cpml(c) == CPML_FUSE_ALL) return bx;
@@ -474,6 +484,16 @@ CpmlBoxes cpml_boxes(const FdtdHipContext *c, int phase) {
       for (auto &z : in[2]) add(xr, fy, z);
     }
   return bx;
+}
+
+// Boxes of both phases.  The separate CPML pass enumerates its cells with 32-bit
+// indices; a region of 2^31 cells or more is refused rather than cut short.
+int set_cpml_boxes(FdtdHipContext *c) {
+  c->cpml_h = cpml_boxes(c, 0);
+  c->cpml_e = cpml_boxes(c, 1);
+  if (c->cpml_h.overflow || c->cpml_e.overflow)
+    return fail("CPML region of 2^31 cells or more: too large for the separate CPML pass (use fuse_cpml 2)");
+  return FDTD_HIP_OK;
 }
 
 bool ntff_blocked(const FdtdHipContext *c) {
@@ -549,13 +569,13 @@ int launch_step(FdtdHipContext *c, int ts, const int *tsp = nullptr) {
   launch_h<T>(c, g, mh, fs);
   if (t) HIP_TRY(hipEventRecord(c->ev[1], c->stream));
   if (c->cpml_h.nbox)
-    hipLaunchKernelGGL(k_cpml_h<T>, dim3((unsigned)((c->cpml_h.start[c->cpml_h.nbox] + 255) / 256)), block, 0,
+    hipLaunchKernelGGL(k_cpml_h<T>, dim3((unsigned)(((long long)c->cpml_h.start[c->cpml_h.nbox] + 255) / 256)), block, 0,
                        c->stream, g, fs, c->cpml_h);
   if (t) HIP_TRY(hipEventRecord(c->ev[2], c->stream));
   launch_e<T>(c, g, me, ghost, fs);
   if (t) HIP_TRY(hipEventRecord(c->ev[3], c->stream));
   if (c->cpml_e.nbox)
-    hipLaunchKernelGGL(k_cpml_e<T>, dim3((unsigned)((c->cpml_e.start[c->cpml_e.nbox] + 255) / 256)), block, 0,
+    hipLaunchKernelGGL(k_cpml_e<T>, dim3((unsigned)(((long long)c->cpml_e.start[c->cpml_e.nbox] + 255) / 256)), block, 0,
                        c->stream, g, fs, c->cpml_e);
   if (t) HIP_TRY(hipEventRecord(c->ev[4], c->stream));
   if (c->ports.nvs + c->ports.nsv + c->ports.nsi > 0) {
@@ -907,8 +927,7 @@ int fdtd_hip_run_batch_async(FdtdHipContext *c, int tsStart, int count) {
     return fail("run_batch: steps [%d,%d) outside [0,%d)", tsStart, tsStart + count, c->d.numberOfTimeSteps);
   if (c->ndft && c->ports.nsv + c->ports.nsi == 0) return fail("run_batch: port DFT without observers");
   if (ensure_traces(c) || finalize_coefficients(c)) return FDTD_HIP_ERR;
-  c->cpml_h = cpml_boxes(c, 0);
-  c->cpml_e = cpml_boxes(c, 1);
+  if (set_cpml_boxes(c)) return FDTD_HIP_ERR;
   if (ntff_blocked(c) && ensure_ntff_buffer(c)) return FDTD_HIP_ERR;
   if (c->fp32() ? run_steps<float>(c, tsStart, count) : run_steps<double>(c, tsStart, count)) return FDTD_HIP_ERR;
   // Leave the accumulators complete at the end of every batch.
@@ -1194,8 +1213,7 @@ int fdtd_hip_autotune(FdtdHipContext *c, int steps_per_candidate, FdtdHipTuning 
   if (steps_per_candidate < 1) steps_per_candidate = 2;
   const int steps = std::min(steps_per_candidate + 1, c->d.numberOfTimeSteps);
   if (ensure_traces(c) || build_coefficient_tables(c)) return FDTD_HIP_ERR;
-  c->cpml_h = cpml_boxes(c, 0);
-  c->cpml_e = cpml_boxes(c, 1);
+  if (set_cpml_boxes(c)) return FDTD_HIP_ERR;
   if (ntff_blocked(c) && ensure_ntff_buffer(c)) return FDTD_HIP_ERR;
   const FdtdHipTuning saved = c->tune;
   const int saved_timing = c->timing;
@@ -1214,8 +1232,7 @@ int fdtd_hip_autotune(FdtdHipContext *c, int steps_per_candidate, FdtdHipTuning 
   FdtdHipTuning best_t = saved;
   for (const auto &t : cands) {
     c->tune = t;
-    c->cpml_h = cpml_boxes(c, 0);
-    c->cpml_e = cpml_boxes(c, 1);
+    if (set_cpml_boxes(c)) return FDTD_HIP_ERR;
     c->coef_mode = -1;
     if (finalize_coefficients(c)) return FDTD_HIP_ERR;
     c->timing = 1;
@@ -1293,17 +1310,17 @@ struct FdtdHipMulti {
   bool fp32 = false;
   hipStream_t stream = nullptr;
   hipEvent_t ev = nullptr;
-  FdtdHipTuning tune{0, 0, 0, CPML_FUSE_ALL};
+  FdtdHipTuning tune{0, 0, 0, CPML_SEPARATE};
   int cm = -1;                  // coefficient storage shared by every variant
   int ntff = 0;                 // 0: no variant has NTFF, 1: direct, 2: blocked
   bool ports = false;
   // Device descriptors and workgroup maps, with the host images last uploaded.
   void *d_desc = nullptr;
-  unsigned *d_map = nullptr;    // [H tiles | E tiles | NTFF blocks]
+  unsigned *d_map = nullptr;    // [H tiles | E tiles | H CPML | E CPML | NTFF blocks]
   size_t desc_cap = 0, map_cap = 0;
   std::vector<char> h_desc;
   std::vector<unsigned> h_map;
-  int nh = 0, ne = 0, nfb = 0;  // H tiles, E tiles, NTFF blocks
+  int nh = 0, ne = 0, nch = 0, nce = 0, nfb = 0;   // H / E tiles, H / E CPML blocks, NTFF blocks
   int *d_step = nullptr;
   hipGraphExec_t graph = nullptr;
   int use_graphs = 1;
@@ -1356,7 +1373,6 @@ int multi_prepare(FdtdHipMulti *m) {
   for (int pass = 0; pass < 2; ++pass) {
     for (FdtdHipContext *c : m->ctx) {
       FdtdHipTuning t = m->tune;
-      t.fuse_cpml = CPML_FUSE_ALL;
       if (pass == 1) t.coefficient_storage = 1;
       if (std::memcmp(&t, &c->tune, sizeof t) != 0) {
         c->tune = t;
@@ -1364,8 +1380,7 @@ int multi_prepare(FdtdHipMulti *m) {
         drop_graph(c);
       }
       if (ensure_traces(c) || finalize_coefficients(c)) return FDTD_HIP_ERR;
-      c->cpml_h = cpml_boxes(c, 0);
-      c->cpml_e = cpml_boxes(c, 1);
+      if (set_cpml_boxes(c)) return FDTD_HIP_ERR;
       if (ntff_blocked(c) && ensure_ntff_buffer(c)) return FDTD_HIP_ERR;
     }
     bool same = true;
@@ -1387,7 +1402,7 @@ int multi_prepare(FdtdHipMulti *m) {
 
   const int n = (int)m->ctx.size();
   std::vector<VariantDesc<T>> desc(n);
-  std::vector<unsigned> hmap, emap, fmap;
+  std::vector<unsigned> hmap, emap, chmap, cemap, fmap;
   for (int v = 0; v < n; ++v) {
     FdtdHipContext *c = m->ctx[v];
     VariantDesc<T> &d = desc[v];
@@ -1397,6 +1412,8 @@ int multi_prepare(FdtdHipMulti *m) {
     d.ghost = c->d.boundary == FDTD_HIP_BOUNDARY_REFERENCE;
     d.mh = march_geom(c, c->d.nx + 1);
     d.me = march_geom(c, d.ghost ? c->d.nx : c->d.nx + 1);
+    d.bh = c->cpml_h;
+    d.be = c->cpml_e;
     d.ps = c->ports;
     d.ps.ndft = c->ndft; d.ps.w = c->dft_w; d.ps.dft = c->dft_acc;
     d.has_ports = c->ports.nvs + c->ports.nsv + c->ports.nsi > 0;
@@ -1408,23 +1425,30 @@ int multi_prepare(FdtdHipMulti *m) {
       const FarfieldState &f = c->ff;
       tf = (2 * ((long)f.nj * f.nk + (long)f.ni * f.nk + (long)f.ni * f.nj) + 255) / 256;
     }
-    if (th > (long)MULTI_LOCAL_MASK || te > (long)MULTI_LOCAL_MASK || tf > (long)MULTI_LOCAL_MASK)
+    const long tch = (long)(((long long)c->cpml_h.start[c->cpml_h.nbox] + 255) / 256);
+    const long tce = (long)(((long long)c->cpml_e.start[c->cpml_e.nbox] + 255) / 256);
+    if (th > (long)MULTI_LOCAL_MASK || te > (long)MULTI_LOCAL_MASK || tf > (long)MULTI_LOCAL_MASK ||
+        tch > (long)MULTI_LOCAL_MASK || tce > (long)MULTI_LOCAL_MASK)
       return fail("multi: variant %d is too large for a batched launch", v);
     put_map<T>(hmap, v, th);
     put_map<T>(emap, v, te);
+    put_map<T>(chmap, v, tch);
+    put_map<T>(cemap, v, tce);
     put_map<T>(fmap, v, tf);
   }
   m->nh = (int)hmap.size();
   m->ne = (int)emap.size();
+  m->nch = (int)chmap.size();
+  m->nce = (int)cemap.size();
   m->nfb = (int)fmap.size();
   std::vector<unsigned> map(hmap);
-  map.insert(map.end(), emap.begin(), emap.end());
-  map.insert(map.end(), fmap.begin(), fmap.end());
+  for (const auto *part : {&emap, &chmap, &cemap, &fmap}) map.insert(map.end(), part->begin(), part->end());
   // The launch choices a captured graph bakes in besides the device images.
   map.push_back((unsigned)m->cm);
   map.push_back((unsigned)cells_per_thread(m->ctx[0]));
   map.push_back((unsigned)m->ntff);
   map.push_back(m->ports ? 1u : 0u);
+  map.push_back((unsigned)m->tune.fuse_cpml);
   const size_t dbytes = desc.size() * sizeof(VariantDesc<T>);
   const bool changed = dbytes != m->h_desc.size() || std::memcmp(desc.data(), m->h_desc.data(), dbytes) != 0 ||
                        map != m->h_map;
@@ -1453,34 +1477,51 @@ int multi_prepare(FdtdHipMulti *m) {
   return FDTD_HIP_OK;
 }
 
-template <typename T, int CM, int W>
+template <typename T, int CM, int W, int FC>
 void multi_launch_hw(FdtdHipMulti *m, bool e_phase) {
   const VariantDesc<T> *vd = (const VariantDesc<T> *)m->d_desc;
   const int nt = e_phase ? m->ne : m->nh;
   const unsigned *map = m->d_map + (e_phase ? m->nh : 0);
   const int per_xcd = (nt + 7) / 8;
   if (e_phase)
-    hipLaunchKernelGGL((k_multi_update_e<T, CM, W, CPML_FUSE_ALL>), dim3(8 * per_xcd), dim3(256), 0, m->stream, vd,
-                       map, nt, per_xcd);
+    hipLaunchKernelGGL((k_multi_update_e<T, CM, W, FC>), dim3(8 * per_xcd), dim3(256), 0, m->stream, vd, map, nt,
+                       per_xcd);
   else
-    hipLaunchKernelGGL((k_multi_update_h<T, CM, W, CPML_FUSE_ALL>), dim3(8 * per_xcd), dim3(256), 0, m->stream, vd,
-                       map, nt, per_xcd);
+    hipLaunchKernelGGL((k_multi_update_h<T, CM, W, FC>), dim3(8 * per_xcd), dim3(256), 0, m->stream, vd, map, nt,
+                       per_xcd);
 }
 
-template <typename T, int CM>
+template <typename T, int CM, int FC>
 void multi_launch_hc(FdtdHipMulti *m, bool e_phase) {
-  switch (cells_per_thread(m->ctx[0])) {
-    case 4: multi_launch_hw<T, CM, 4>(m, e_phase); break;
-    case 2: multi_launch_hw<T, CM, 2>(m, e_phase); break;
-    default: multi_launch_hw<T, CM, 1>(m, e_phase); break;
+  const int w = cells_per_thread(m->ctx[0]);
+  if constexpr (sizeof(T) == 4) {
+    if (w == 4) { multi_launch_hw<T, CM, 4, FC>(m, e_phase); return; }
   }
+  if (w == 2) multi_launch_hw<T, CM, 2, FC>(m, e_phase);
+  else multi_launch_hw<T, CM, 1, FC>(m, e_phase);
 }
 
+template <typename T, int FC>
+void multi_launch_hf(FdtdHipMulti *m, bool e_phase) {
+  if (m->cm == COEF_U8) multi_launch_hc<T, COEF_U8, FC>(m, e_phase);
+  else if (m->cm == COEF_U16) multi_launch_hc<T, COEF_U16, FC>(m, e_phase);
+  else multi_launch_hc<T, COEF_ARRAYS, FC>(m, e_phase);
+}
+
+// The stencil of one phase for every variant, then its CPML pass when one is left.
 template <typename T>
 void multi_launch_phase(FdtdHipMulti *m, bool e_phase) {
-  if (m->cm == COEF_U8) multi_launch_hc<T, COEF_U8>(m, e_phase);
-  else if (m->cm == COEF_U16) multi_launch_hc<T, COEF_U16>(m, e_phase);
-  else multi_launch_hc<T, COEF_ARRAYS>(m, e_phase);
+  switch (m->tune.fuse_cpml) {
+    case CPML_FUSE_ALL: multi_launch_hf<T, CPML_FUSE_ALL>(m, e_phase); break;
+    case CPML_FUSE_Z: multi_launch_hf<T, CPML_FUSE_Z>(m, e_phase); break;
+    default: multi_launch_hf<T, CPML_SEPARATE>(m, e_phase); break;
+  }
+  const VariantDesc<T> *vd = (const VariantDesc<T> *)m->d_desc;
+  const int nb = e_phase ? m->nce : m->nch;
+  const unsigned *map = m->d_map + m->nh + m->ne + (e_phase ? m->nch : 0);
+  if (nb == 0) return;
+  if (e_phase) hipLaunchKernelGGL(k_multi_cpml_e<T>, dim3((unsigned)nb), dim3(256), 0, m->stream, vd, map);
+  else hipLaunchKernelGGL(k_multi_cpml_h<T>, dim3((unsigned)nb), dim3(256), 0, m->stream, vd, map);
 }
 
 // One time step of every variant (see launch_step for ts / tsp).
@@ -1493,14 +1534,14 @@ int multi_step(FdtdHipMulti *m, int ts, const int *tsp = nullptr) {
     hipLaunchKernelGGL(k_multi_port_epilogue<T>, dim3((unsigned)m->ctx.size()), dim3(256), 0, m->stream, vd, tsp, ts);
   if (m->ntff == 1) {
     hipLaunchKernelGGL(k_multi_ntff_direct<T>, dim3((unsigned)m->nfb), dim3(256), 0, m->stream, vd,
-                       m->d_map + m->nh + m->ne, tsp, ts);
+                       m->d_map + m->nh + m->ne + m->nch + m->nce, tsp, ts);
   } else if (m->ntff == 2) {
     // Every variant's block state moves in lockstep (multi_run restarts them together).
     FdtdHipContext *lead = nullptr;
     for (FdtdHipContext *c : m->ctx) if (c->nff) { lead = c; break; }
     const int row = tsp ? ts : lead->blk_rows;
     hipLaunchKernelGGL(k_multi_ntff_gather<T>, dim3((unsigned)m->nfb), dim3(256), 0, m->stream, vd,
-                       m->d_map + m->nh + m->ne, row);
+                       m->d_map + m->nh + m->ne + m->nch + m->nce, row);
     for (FdtdHipContext *c : m->ctx) {
       if (!c->nff) continue;
       if (tsp) {
@@ -1622,12 +1663,9 @@ int fdtd_hip_multi_create(FdtdHipContext *const *ctxs, int n, const FdtdHipTunin
     if (ctxs[v]->device != ctxs[0]->device) return fail("multi_create: variants on different devices");
     if (ctxs[v]->fp32() != ctxs[0]->fp32()) return fail("multi_create: variants mix FP32 and FP64");
   }
-  FdtdHipTuning t{0, 0, 0, CPML_FUSE_ALL};
+  FdtdHipTuning t{0, 0, 0, CPML_SEPARATE};
   if (tuning) {
     t = *tuning;
-    if (t.fuse_cpml != 0 && t.fuse_cpml != CPML_FUSE_ALL)
-      return fail("multi_create: batched launches fuse every CPML face (fuse_cpml 0 or 2)");
-    t.fuse_cpml = CPML_FUSE_ALL;
     if (check_tuning(ctxs[0], &t, "multi_create")) return FDTD_HIP_ERR;
   }
   HIP_TRY(hipSetDevice(ctxs[0]->device));
@@ -1680,7 +1718,8 @@ int fdtd_hip_multi_autotune(FdtdHipMulti *m, int steps_per_candidate, FdtdHipTun
   for (int cs = 0; cs < 2; ++cs)
     for (int w : {1, 2, 4}) {
       if (w == 4 && !m->fp32) continue;
-      for (int pl : {1, 2, 4, 8}) cands.push_back(FdtdHipTuning{pl, w, cs, CPML_FUSE_ALL});
+      for (int pl : {1, 2, 4, 8})
+        for (int fc = CPML_SEPARATE; fc <= CPML_FUSE_ALL; ++fc) cands.push_back(FdtdHipTuning{pl, w, cs, fc});
     }
   hipEvent_t e0 = nullptr, e1 = nullptr;
   HIP_TRY(hipSetDevice(m->device));

@@ -308,24 +308,30 @@ __device__ __forceinline__ void update_h_body(
     coeffs<T, CM, W>(g.ct, tc, 0, 0, c, g.C[CHXH], g.C[CHXEY], g.C[CHXEZ], c0, c1, c2);
     coeffs<T, CM, W>(g.ct, tc, 0, 1, c, g.C[CHYH], g.C[CHYEZ], g.C[CHYEX], c3, c4, c5);
     coeffs<T, CM, W>(g.ct, tc, 0, 2, c, g.C[CHZH], g.C[CHZEX], g.C[CHZEY], c6, c7, c8);
+    // The six E differences of the curls; the CPML terms use the same ones (E does not
+    // change during the H phase).
+    V dy_hx, dz_hx, dx_hy, dz_hy, dx_hz, dy_hz;
 #pragma unroll
     for (int e = 0; e < W; ++e) {
       const int k = k0 + e;
       const T ey_k = e + 1 < W ? ey_i[e + 1 < W ? e + 1 : 0] : ey_w;
       const T ex_k = e + 1 < W ? ex_i[e + 1 < W ? e + 1 : 0] : ex_w;
+      dz_hx[e] = ey_k - ey_i[e];       // Hx: dEy along z
+      dy_hx[e] = ez_j[e] - ez_i[e];    //     dEz along y
+      dx_hy[e] = ez_n[e] - ez_i[e];    // Hy: dEz along x
+      dz_hy[e] = ex_k - ex_i[e];       //     dEx along z
+      dy_hz[e] = ex_j[e] - ex_i[e];    // Hz: dEx along y
+      dx_hz[e] = ey_n[e] - ey_i[e];    //     dEy along x
       if (j < ny && k < nz) {                              // Hx (nx+1, ny, nz)
-        const T dEy = ey_k - ey_i[e], dEz = ez_j[e] - ez_i[e];
-        T h = c0[e] * hx[e] + c1[e] * dEy + c2[e] * dEz;
+        T h = c0[e] * hx[e] + c1[e] * dz_hx[e] + c2[e] * dy_hx[e];
         hx[e] = h;
       }
       if (inner && k < nz) {                               // Hy (nx, ny+1, nz)
-        const T dEz = ez_n[e] - ez_i[e], dEx = ex_k - ex_i[e];
-        T h = c3[e] * hy[e] + c4[e] * dEz + c5[e] * dEx;
+        T h = c3[e] * hy[e] + c4[e] * dx_hy[e] + c5[e] * dz_hy[e];
         hy[e] = h;
       }
       if (inner && j < ny && k <= nz) {                    // Hz (nx, ny, nz+1)
-        const T dEx = ex_j[e] - ex_i[e], dEy = ey_n[e] - ey_i[e];
-        T h = c6[e] * hz[e] + c7[e] * dEx + c8[e] * dEy;
+        T h = c6[e] * hz[e] + c7[e] * dy_hz[e] + c8[e] * dx_hz[e];
         hz[e] = h;
       }
     }
@@ -334,18 +340,18 @@ __device__ __forceinline__ void update_h_body(
       // bulk update and in the reference's face order; k_cpml skips them.
 #pragma unroll
       for (int e = 0; e < W; ++e) {
-        const int k = k0 + e, ce = c + e;
+        const int k = k0 + e;
         if (k > nz || !fused_cpml_cell<T, FC, true>(fs, i, j, k)) continue;
         bool hit;
         if (j < ny && k < nz)                              // Hx: y then z faces
           hx[e] = cpml_terms<T, 1, 0, 2, 0, true>(fs, hx[e], i, j, k, ny, nz,
-              [&] { return Ez[ce + SJ] - Ez[ce]; }, [&] { return Ey[ce + 1] - Ey[ce]; }, &hit);
+              [&] { return dy_hx[e]; }, [&] { return dz_hx[e]; }, &hit);
         if (inner && k < nz)                               // Hy: x then z faces
           hy[e] = cpml_terms<T, 0, 0, 2, 1, true>(fs, hy[e], i, j, k, ny + 1, nz,
-              [&] { return Ez[ce + SI] - Ez[ce]; }, [&] { return Ex[ce + 1] - Ex[ce]; }, &hit);
+              [&] { return dx_hy[e]; }, [&] { return dz_hy[e]; }, &hit);
         if (inner && j < ny)                               // Hz: x then y faces
           hz[e] = cpml_terms<T, 0, 1, 1, 1, true>(fs, hz[e], i, j, k, ny, nz + 1,
-              [&] { return Ey[ce + SI] - Ey[ce]; }, [&] { return Ex[ce + SJ] - Ex[ce]; }, &hit);
+              [&] { return dx_hz[e]; }, [&] { return dy_hz[e]; }, &hit);
       }
     }
     stv<T, W>(Hx + c, hx);
@@ -401,32 +407,38 @@ __device__ __forceinline__ void update_e_body(
     coeffs<T, CM, W>(g.ct, tc, 3, 1, c, g.C[CEYE], g.C[CEYHX], g.C[CEYHZ], c3, c4, c5);
     coeffs<T, CM, W>(g.ct, tc, 3, 2, c, g.C[CEZE], g.C[CEZHY], g.C[CEZHX], c6, c7, c8);
     V exb = ex;
+    // The six H differences of the curls, shared with the CPML terms (H does not change
+    // during the E phase).
+    V dy_ex, dz_ex, dz_ey, dx_ey, dx_ez, dy_ez;
 #pragma unroll
     for (int e = 0; e < W; ++e) {
       const int k = k0 + e;
       const bool kin = k >= 1 && k < nz;
       const T hy_k = e > 0 ? hy_i[e > 0 ? e - 1 : 0] : hy_w;
       const T hx_k = e > 0 ? hx_i[e > 0 ? e - 1 : 0] : hx_w;
+      dy_ex[e] = hz_i[e] - hz_j[e];    // Ex: dHz along y
+      dz_ex[e] = hy_i[e] - hy_k;       //     dHy along z
+      dz_ey[e] = hx_i[e] - hx_k;       // Ey: dHx along z
+      dx_ey[e] = hz_i[e] - hz_m[e];    //     dHz along x
+      dx_ez[e] = hy_i[e] - hy_m[e];    // Ez: dHy along x
+      dy_ez[e] = hx_i[e] - hx_j[e];    //     dHx along y
       if (i < nx && k <= nz) {                             // Ex (nx, ny+1, nz+1)
-        const T dHz = hz_i[e] - hz_j[e], dHy = hy_i[e] - hy_k;
         bool t = jin && kin;
         T v = ex[e];
-        if (t) v = c0[e] * v + c1[e] * dHz + c2[e] * dHy;
+        if (t) v = c0[e] * v + c1[e] * dy_ex[e] + c2[e] * dz_ex[e];
         exb[e] = v;
         ex[e] = v;
       }
       if (j < ny && k <= nz) {                             // Ey (nx+1, ny, nz+1)
-        const T dHx = hx_i[e] - hx_k, dHz = hz_i[e] - hz_m[e];
         bool t = iin && kin;
         T v = ey[e];
-        if (t) v = c3[e] * v + c4[e] * dHx + c5[e] * dHz;
+        if (t) v = c3[e] * v + c4[e] * dz_ey[e] + c5[e] * dx_ey[e];
         ey[e] = v;
       }
       if (k < nz) {                                        // Ez (nx+1, ny+1, nz)
-        const T dHy = hy_i[e] - hy_m[e], dHx = hx_i[e] - hx_j[e];
         bool t = iin && jin;
         T v = ez[e];
-        if (t) v = c6[e] * v + c7[e] * dHy + c8[e] * dHx;
+        if (t) v = c6[e] * v + c7[e] * dx_ez[e] + c8[e] * dy_ez[e];
         ez[e] = v;
       }
     }
@@ -434,18 +446,18 @@ __device__ __forceinline__ void update_e_body(
       // As in update_h_body; exb keeps the pre-CPML Ex that the ghost plane reads.
 #pragma unroll
       for (int e = 0; e < W; ++e) {
-        const int k = k0 + e, ce = c + e;
+        const int k = k0 + e;
         if (k > nz || !fused_cpml_cell<T, FC, false>(fs, i, j, k)) continue;
         bool hit;
         if (i < nx)                                        // Ex: y then z faces
           ex[e] = cpml_terms<T, 1, 3, 2, 2, false>(fs, ex[e], i, j, k, ny + 1, nz + 1,
-              [&] { return Hz[ce] - Hz[ce - SJ]; }, [&] { return Hy[ce] - Hy[ce - 1]; }, &hit);
+              [&] { return dy_ex[e]; }, [&] { return dz_ex[e]; }, &hit);
         if (j < ny)                                        // Ey: x then z faces
           ey[e] = cpml_terms<T, 0, 2, 2, 3, false>(fs, ey[e], i, j, k, ny, nz + 1,
-              [&] { return Hz[ce] - Hz[ce - SI]; }, [&] { return Hx[ce] - Hx[ce - 1]; }, &hit);
+              [&] { return dx_ey[e]; }, [&] { return dz_ey[e]; }, &hit);
         if (k < nz)                                        // Ez: x then y faces
           ez[e] = cpml_terms<T, 0, 3, 1, 2, false>(fs, ez[e], i, j, k, ny + 1, nz,
-              [&] { return Hy[ce] - Hy[ce - SI]; }, [&] { return Hx[ce] - Hx[ce - SJ]; }, &hit);
+              [&] { return dx_ez[e]; }, [&] { return dy_ez[e]; }, &hit);
       }
     }
     if (i < nx) stv<T, W>(Ex + c, ex);
@@ -507,24 +519,34 @@ __global__ void __launch_bounds__(256) k_update_e(Grid<T> g, MarchGeom m, int gh
 
 // The cells any CPML face of a phase touches, as disjoint boxes: the x-slabs over the
 // full y,z extent, then the y-slabs outside the x-slabs, then the z-slabs outside
-// both.  Boxes are enumerated k-fastest through a flat index.
+// both.  Boxes are enumerated k-fastest through a flat index, one cell per thread.
+// (Four cells per thread with every load issued before the first store was measured
+// slower: its registers halved the occupancy.)
 constexpr int MAX_CPML_BOXES = 32;
 struct CpmlBoxes {
   int nbox;
-  long long start[MAX_CPML_BOXES + 1];   // flat start of each box; start[nbox] = total cells
+  int start[MAX_CPML_BOXES + 1];   // flat start of each box; start[nbox] = total cells (< 2^31)
   int lo[MAX_CPML_BOXES][3], ext[MAX_CPML_BOXES][3];
+  int overflow;                    // set by the host builder when the cells exceed 2^31 - 1
 };
 
+// Box coordinates of flat index q, in 32-bit unsigned arithmetic (a 64-bit division
+// costs several times as many instructions).  The box search compares against every
+// start with constant indices, so the starts sit in scalar registers and no load
+// waits on another.
 __device__ __forceinline__ bool cpml_cell(const CpmlBoxes &bx, long long q, int &i, int &j, int &k) {
-  if (q >= bx.start[bx.nbox]) return false;
+  const int nbox = bx.nbox;
+  if (q >= bx.start[nbox]) return false;
+  const unsigned u = (unsigned)q;
   int b = 0;
-  while (q >= bx.start[b + 1]) ++b;
-  long long r = q - bx.start[b];
-  const int ek = bx.ext[b][2], ej = bx.ext[b][1];
-  k = bx.lo[b][2] + (int)(r % ek);
-  r /= ek;
-  j = bx.lo[b][1] + (int)(r % ej);
-  i = bx.lo[b][0] + (int)(r / ej);
+#pragma unroll
+  for (int t = 1; t < MAX_CPML_BOXES; ++t) b += (t < nbox && u >= (unsigned)bx.start[t]) ? 1 : 0;
+  const unsigned r = u - (unsigned)bx.start[b];
+  const unsigned ek = (unsigned)bx.ext[b][2], ej = (unsigned)bx.ext[b][1];
+  const unsigned rk = r / ek, rj = rk / ej;
+  k = bx.lo[b][2] + (int)(r - rk * ek);
+  j = bx.lo[b][1] + (int)(rk - rj * ej);
+  i = bx.lo[b][0] + (int)rj;
   return true;
 }
 
@@ -533,10 +555,10 @@ __device__ __forceinline__ bool cpml_cell(const CpmlBoxes &bx, long long q, int 
 // update, as the reference's updateMagneticCPML does; the E differences are this
 // step's E, which no kernel changes in between.
 template <typename T>
-__global__ void __launch_bounds__(256) k_cpml_h(Grid<T> g, FaceSet<T> fs, CpmlBoxes bx) {
+__device__ __forceinline__ void cpml_h_body(const Grid<T> &g, const FaceSet<T> &fs, const CpmlBoxes &bx, long long q) {
 #pragma clang fp contract(off)
   int i, j, k;
-  if (!cpml_cell(bx, (long long)blockIdx.x * blockDim.x + threadIdx.x, i, j, k)) return;
+  if (!cpml_cell(bx, q, i, j, k)) return;
   const int nx = g.nx, ny = g.ny, nz = g.nz, SJ = g.SJ, SI = g.SI;
   const int c = i * SI + j * SJ + k;
   const T *Ex = g.Ex, *Ey = g.Ey, *Ez = g.Ez;
@@ -555,10 +577,10 @@ __global__ void __launch_bounds__(256) k_cpml_h(Grid<T> g, FaceSet<T> fs, CpmlBo
 // E extents, so the i = nx wall of Ey/Ez gets its z / y terms here whichever
 // boundary mode updated (or skipped) its bulk value.
 template <typename T>
-__global__ void __launch_bounds__(256) k_cpml_e(Grid<T> g, FaceSet<T> fs, CpmlBoxes bx) {
+__device__ __forceinline__ void cpml_e_body(const Grid<T> &g, const FaceSet<T> &fs, const CpmlBoxes &bx, long long q) {
 #pragma clang fp contract(off)
   int i, j, k;
-  if (!cpml_cell(bx, (long long)blockIdx.x * blockDim.x + threadIdx.x, i, j, k)) return;
+  if (!cpml_cell(bx, q, i, j, k)) return;
   const int nx = g.nx, ny = g.ny, nz = g.nz, SJ = g.SJ, SI = g.SI;
   const int c = i * SI + j * SJ + k;
   const T *Hx = g.Hx, *Hy = g.Hy, *Hz = g.Hz;
@@ -571,6 +593,16 @@ __global__ void __launch_bounds__(256) k_cpml_e(Grid<T> g, FaceSet<T> fs, CpmlBo
   if (k < nz)                                              // Ez: x then y faces
     cpml_component<T, 0, 3, 1, 2, false>(fs, g.Ez, c, i, j, k, ny + 1, nz,
         [&] { return Hy[c] - Hy[c - SI]; }, [&] { return Hx[c] - Hx[c - SJ]; });
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_cpml_h(Grid<T> g, FaceSet<T> fs, CpmlBoxes bx) {
+  cpml_h_body<T>(g, fs, bx, (long long)blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_cpml_e(Grid<T> g, FaceSet<T> fs, CpmlBoxes bx) {
+  cpml_e_body<T>(g, fs, bx, (long long)blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 // ---------------------------------------------------------------- port epilogue --

@@ -10,8 +10,8 @@
 // Stencil tiles of all variants are numbered variant-major, column-fastest within a
 // variant, and dealt to the 8 XCDs in contiguous ranges exactly as march_tile deals
 // one problem's tiles, so neighbouring columns of a variant still share an L2.  The
-// CPML terms are always fused into the stencil kernels (CPML_FUSE_ALL): a separate
-// CPML pass would be one more launch per phase.
+// CPML placement (separate pass, z slabs fused, all faces fused) is one choice for
+// the whole batch; the separate pass is then one more batched launch per phase.
 #pragma once
 
 namespace fdtd {
@@ -21,6 +21,7 @@ struct VariantDesc {
   Grid<T> g;
   FaceSet<T> fs;
   MarchGeom mh, me;
+  CpmlBoxes bh, be;     // cells left to the CPML pass (none when every face is fused)
   int ghost;
   int has_ports;
   PortState ps;
@@ -109,6 +110,25 @@ __global__ void __launch_bounds__(256) k_multi_update_e(const VariantDesc<T> *__
   const FaceSet<T> fs = global_faces(d.fs);
   load_tables<T, CM>(g, 3, tc);
   update_e_body<T, CM, W, FC>(g.Ex, g.Ey, g.Ez, g.Hx, g.Hy, g.Hz, g, d.me, tc, col, ch, d.ghost, fs);
+}
+
+// The CPML pass: each variant's box cells padded to whole blocks, as for the NTFF.
+template <typename T>
+__global__ void __launch_bounds__(256) k_multi_cpml_h(const VariantDesc<T> *__restrict__ vd,
+                                                      const unsigned *__restrict__ map) {
+  const unsigned entry = map[blockIdx.x];
+  const VariantDesc<T> &d = vd[entry >> MULTI_LOCAL_BITS];
+  const long long q = (long long)(entry & MULTI_LOCAL_MASK) * blockDim.x + threadIdx.x;
+  cpml_h_body<T>(global_grid(d.g), global_faces(d.fs), d.bh, q);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_multi_cpml_e(const VariantDesc<T> *__restrict__ vd,
+                                                      const unsigned *__restrict__ map) {
+  const unsigned entry = map[blockIdx.x];
+  const VariantDesc<T> &d = vd[entry >> MULTI_LOCAL_BITS];
+  const long long q = (long long)(entry & MULTI_LOCAL_MASK) * blockDim.x + threadIdx.x;
+  cpml_e_body<T>(global_grid(d.g), global_faces(d.fs), d.be, q);
 }
 
 // One block per variant; variants without sources or observers return at once.

@@ -255,12 +255,13 @@ class HipMulti:
     Each context's results equal its own run with the same tuning, bit for bit."""
 
     def __init__(self, sims, planes_per_workgroup: int = 0, cells_per_thread: int = 0,
-                 coefficient_storage: int = 0):
+                 coefficient_storage: int = 0, fuse_cpml: int = 0):
         self.sims = list(sims)
         self.L = native.lib()
         self.h = C.c_void_p()
         arr = (C.c_void_p * len(self.sims))(*[s.h for s in self.sims])
-        t = native.FdtdHipTuning(int(planes_per_workgroup), int(cells_per_thread), int(coefficient_storage), 2)
+        t = native.FdtdHipTuning(int(planes_per_workgroup), int(cells_per_thread), int(coefficient_storage),
+                                 int(fuse_cpml))
         native.check(self.L.fdtd_hip_multi_create(arr, len(self.sims), C.byref(t), C.byref(self.h)), "multi_create")
 
     def close(self):

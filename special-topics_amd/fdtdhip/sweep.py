@@ -157,10 +157,10 @@ class Sweep:
             raise ValueError("a batched group needs one step count")
         if self.batch_tuning is None:
             if self.tuning is not None:
-                self.batch_tuning = tuple(self.tuning[:3])
+                self.batch_tuning = tuple(self.tuning) + (0,) * (4 - len(self.tuning))
             else:
                 m = HipMulti([j["sim"] for j in jobs])
-                self.batch_tuning = m.autotune(2)[:3]
+                self.batch_tuning = m.autotune(2)
                 m.close()
         multi = HipMulti([j["sim"] for j in jobs], *self.batch_tuning)
         multi.run_async(0, steps.pop())
@@ -236,10 +236,15 @@ def main():
     ap.add_argument("--boundary", default="matlab", choices=["matlab", "reference"])
     ap.add_argument("--inflight", type=int, default=4)
     ap.add_argument("--batch", type=int, default=64, help="variants per batched launch (0: streams in flight)")
+    ap.add_argument("--tuning", default=None,
+                    help="planes,cells,storage,cpml for every launch (default: the batch autotuner)")
+    ap.add_argument("--steps", type=int, default=None, help="steps per variant (default: the scenario's)")
     ap.add_argument("--out", default=None)
     a = ap.parse_args()
-    files = build_all(a.scenarios, a.workdir, fp32=a.precision == "fp32")
-    res, secs, cells = Sweep(a.precision, a.boundary, a.inflight, keep_traces=False, batch=a.batch).run(files)
+    tuning = tuple(int(x) for x in a.tuning.split(",")) if a.tuning else None
+    files = build_all(a.scenarios, a.workdir, fp32=a.precision == "fp32", jobs=16)
+    res, secs, cells = Sweep(a.precision, a.boundary, a.inflight, tuning=tuning, keep_traces=False,
+                             steps=a.steps, batch=a.batch).run(files)
     summary = {"variants": len(res), "seconds": round(secs, 3), "mcell_updates_per_s": round(cells / secs / 1e6, 1),
                "results": [{k: v for k, v in r.items() if k not in ("s11", "s11_db", "cuts", "v", "i")} for r in res]}
     text = json.dumps(summary)

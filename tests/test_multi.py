@@ -67,19 +67,19 @@ def _assert_same(a, b, label):
 
 
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
-@pytest.mark.parametrize("cells,graphs", [(1, True), (2, False)])
-def test_batched_run_matches_each_single_run(variants, precision, cells, graphs):
+@pytest.mark.parametrize("cells,graphs,fuse", [(1, True, 2), (2, False, 2), (1, True, 0), (2, True, 1)])
+def test_batched_run_matches_each_single_run(variants, precision, cells, graphs, fuse):
     from fdtdhip.sim import HipMulti
     probs = variants[precision == "fp32"]
     ref = []
     for sim in _sims(probs, precision):
-        sim.set_tuning(0, cells, 0, 2)
+        sim.set_tuning(0, cells, 0, fuse)
         sim.set_graphs(graphs)
         tr = [sim.run(0, SPLIT), sim.run(SPLIT, STEPS - SPLIT)]
         ref.append(_state(sim, tr))
         sim.close()
     sims = _sims(probs, precision)
-    multi = HipMulti(sims, cells_per_thread=cells)
+    multi = HipMulti(sims, cells_per_thread=cells, fuse_cpml=fuse)
     multi.set_graphs(graphs)
     first = multi.run(0, SPLIT)
     second = multi.run(SPLIT, STEPS - SPLIT)
@@ -98,14 +98,14 @@ def test_batched_autotune_then_run_matches_single(variants):
     sims = _sims(probs, "fp32")
     multi = HipMulti(sims)
     planes, cells, storage, fuse = multi.autotune(2)
-    assert planes >= 1 and cells in (1, 2, 4) and storage in (0, 1) and fuse == 2
+    assert planes >= 1 and cells in (1, 2, 4) and storage in (0, 1) and fuse in (0, 1, 2)
     got = multi.run(0, STEPS)
     states = [_state(s, [got[n]]) for n, s in enumerate(sims)]
     multi.close()
     for s in sims:
         s.close()
     for n, sim in enumerate(_sims(probs, "fp32")):
-        sim.set_tuning(planes, cells, storage, 2)
+        sim.set_tuning(planes, cells, storage, fuse)
         _assert_same(states[n], _state(sim, [sim.run(0, STEPS)]), f"variant {n} after autotune")
         sim.close()
 
