@@ -519,9 +519,11 @@ __global__ void __launch_bounds__(256) k_update_e(Grid<T> g, MarchGeom m, int gh
 
 // The cells any CPML face of a phase touches, as disjoint boxes: the x-slabs over the
 // full y,z extent, then the y-slabs outside the x-slabs, then the z-slabs outside
-// both.  Boxes are enumerated k-fastest through a flat index, one cell per thread.
-// (Four cells per thread with every load issued before the first store was measured
-// slower: its registers halved the occupancy.)
+// both.  Boxes are enumerated k-fastest through a flat index, one cell per thread,
+// one component after another.  Two restructurings were measured slower, both through
+// the occupancy their registers cost: four cells per thread with each component's
+// loads issued before its stores, and all three components' loads issued before any
+// store.
 constexpr int MAX_CPML_BOXES = 32;
 struct CpmlBoxes {
   int nbox;
@@ -531,16 +533,12 @@ struct CpmlBoxes {
 };
 
 // Box coordinates of flat index q, in 32-bit unsigned arithmetic (a 64-bit division
-// costs several times as many instructions).  The box search compares against every
-// start with constant indices, so the starts sit in scalar registers and no load
-// waits on another.
+// costs several times as many instructions).
 __device__ __forceinline__ bool cpml_cell(const CpmlBoxes &bx, long long q, int &i, int &j, int &k) {
-  const int nbox = bx.nbox;
-  if (q >= bx.start[nbox]) return false;
+  if (q >= bx.start[bx.nbox]) return false;
   const unsigned u = (unsigned)q;
   int b = 0;
-#pragma unroll
-  for (int t = 1; t < MAX_CPML_BOXES; ++t) b += (t < nbox && u >= (unsigned)bx.start[t]) ? 1 : 0;
+  while (u >= (unsigned)bx.start[b + 1]) ++b;
   const unsigned r = u - (unsigned)bx.start[b];
   const unsigned ek = (unsigned)bx.ext[b][2], ej = (unsigned)bx.ext[b][1];
   const unsigned rk = r / ek, rj = rk / ej;
