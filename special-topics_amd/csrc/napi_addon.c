@@ -31,10 +31,13 @@
  *   multiRun(m, tsStart, count); multiSynchronize(m)       problems, one launch per
  *   multiAutotune(m, stepsPerCandidate) -> tuning          phase)
  *   multiDestroy(m)
+ * Handles stay valid until destroyed (a second destroy throws); destroy a batch before
+ * the problems it steps.
  */
 #define NAPI_VERSION 8
 #include <node_api.h>
 #include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include "../../include/fdtd_hip.h"
@@ -124,13 +127,57 @@ static int prop_double(napi_env env, napi_value obj, const char *key, double *ou
   return prop(env, obj, key, &v) && get_double(env, v, out);
 }
 
-static FdtdHipContext *handle(napi_env env, napi_value v) {
-  void *p = NULL;
-  if (napi_get_value_external(env, v, &p) != napi_ok || !p) {
-    napi_throw_type_error(env, NULL, "expected a live problem handle");
+/* A handle is an external around a HandleBox.  destroy()/multiDestroy() free the
+ * library object and clear the box, so a second call on the same handle throws
+ * instead of touching freed memory; the box itself goes with the JS value, and a
+ * handle collected without destroy() frees its library object then. */
+typedef struct {
+  void *p;
+  int kind;    /* 1 = problem (FdtdHipContext), 2 = batch (FdtdHipMulti) */
+} HandleBox;
+
+static void box_finalize(napi_env env, void *data, void *hint) {
+  (void)env; (void)hint;
+  HandleBox *b = (HandleBox *)data;
+  if (b->p) {
+    if (b->kind == 1) fdtd_hip_destroy((FdtdHipContext *)b->p);
+    else fdtd_hip_multi_destroy((FdtdHipMulti *)b->p);
+  }
+  free(b);
+}
+
+static napi_value make_handle(napi_env env, void *p, int kind) {
+  HandleBox *b = (HandleBox *)malloc(sizeof *b);
+  napi_value ext;
+  if (!b) {
+    napi_throw_error(env, NULL, "out of host memory");
     return NULL;
   }
-  return (FdtdHipContext *)p;
+  b->p = p;
+  b->kind = kind;
+  if (napi_create_external(env, b, box_finalize, NULL, &ext) != napi_ok) {
+    b->p = NULL;
+    free(b);
+    throw_napi(env, "napi_create_external");
+    return NULL;
+  }
+  return ext;
+}
+
+static HandleBox *box_of(napi_env env, napi_value v, int kind) {
+  void *p = NULL;
+  napi_valuetype vt;
+  if (napi_typeof(env, v, &vt) != napi_ok || vt != napi_external || napi_get_value_external(env, v, &p) != napi_ok ||
+      !p || ((HandleBox *)p)->kind != kind || !((HandleBox *)p)->p) {
+    napi_throw_type_error(env, NULL, kind == 1 ? "expected a live problem handle" : "expected a live batch handle");
+    return NULL;
+  }
+  return (HandleBox *)p;
+}
+
+static FdtdHipContext *handle(napi_env env, napi_value v) {
+  HandleBox *b = box_of(env, v, 1);
+  return b ? (FdtdHipContext *)b->p : NULL;
 }
 
 static napi_value DeviceCount(napi_env env, napi_callback_info info) {
@@ -156,8 +203,8 @@ static napi_value Init(napi_env env, napi_callback_info info) {
     return NULL;
   FdtdHipContext *ctx = NULL;
   if (fdtd_hip_init(&d, &ctx) != FDTD_HIP_OK) return throw_lib(env, "init");
-  napi_value ext;
-  CALL(env, napi_create_external(env, ctx, NULL, NULL, &ext));
+  napi_value ext = make_handle(env, ctx, 1);
+  if (!ext) fdtd_hip_destroy(ctx);
   return ext;
 }
 
@@ -486,9 +533,10 @@ static napi_value Autotune(napi_env env, napi_callback_info info) {
 static napi_value Destroy(napi_env env, napi_callback_info info) {
   napi_value argv[1];
   if (!get_args(env, info, 1, argv)) return NULL;
-  FdtdHipContext *c = handle(env, argv[0]);
-  if (!c) return NULL;
-  fdtd_hip_destroy(c);
+  HandleBox *b = box_of(env, argv[0], 1);
+  if (!b) return NULL;
+  fdtd_hip_destroy((FdtdHipContext *)b->p);
+  b->p = NULL;
   return undefined(env);
 }
 
@@ -506,18 +554,9 @@ static napi_value ReadTraces(napi_env env, napi_callback_info info) {
   return undefined(env);
 }
 
-/* Batch handles carry a type tag so a problem handle is never taken for one. */
-static const napi_type_tag kMultiTag = {0x6664746468697030ULL, 0x6d756c7469626174ULL};
-
 static FdtdHipMulti *multi_handle(napi_env env, napi_value v) {
-  bool is = false;
-  void *p = NULL;
-  if (napi_check_object_type_tag(env, v, &kMultiTag, &is) != napi_ok || !is ||
-      napi_get_value_external(env, v, &p) != napi_ok || !p) {
-    napi_throw_type_error(env, NULL, "expected a live batch handle");
-    return NULL;
-  }
-  return (FdtdHipMulti *)p;
+  HandleBox *b = box_of(env, v, 2);
+  return b ? (FdtdHipMulti *)b->p : NULL;
 }
 
 static int tuning_from(napi_env env, napi_value obj, FdtdHipTuning *t) {
@@ -559,13 +598,8 @@ static napi_value MultiCreate(napi_env env, napi_callback_info info) {
   }
   FdtdHipMulti *m = NULL;
   if (fdtd_hip_multi_create(ctx, (int)n, tp, &m) != FDTD_HIP_OK) return throw_lib(env, "multiCreate");
-  napi_value ext;
-  if (napi_create_external(env, m, NULL, NULL, &ext) != napi_ok ||
-      napi_type_tag_object(env, ext, &kMultiTag) != napi_ok) {
-    fdtd_hip_multi_destroy(m);
-    throw_napi(env, "multiCreate");
-    return NULL;
-  }
+  napi_value ext = make_handle(env, m, 2);
+  if (!ext) fdtd_hip_multi_destroy(m);
   return ext;
 }
 
@@ -602,9 +636,10 @@ static napi_value MultiAutotune(napi_env env, napi_callback_info info) {
 static napi_value MultiDestroy(napi_env env, napi_callback_info info) {
   napi_value argv[1];
   if (!get_args(env, info, 1, argv)) return NULL;
-  FdtdHipMulti *m = multi_handle(env, argv[0]);
-  if (!m) return NULL;
-  fdtd_hip_multi_destroy(m);
+  HandleBox *b = box_of(env, argv[0], 2);
+  if (!b) return NULL;
+  fdtd_hip_multi_destroy((FdtdHipMulti *)b->p);
+  b->p = NULL;
   return undefined(env);
 }
 

@@ -235,3 +235,31 @@ def test_hip_backend_run_many_matches_run(tmp_path, precision):
         diff = [k for k in a if a[k] != b.get(k)]
         diff += [f"{k}.{q}" for k in ("fields", "ff") for q in a[k] if a[k][q] != b[k].get(q)]
         assert a == b, f"scenario {scen[n]}: runMany differs from run in {diff}"
+
+
+@pytest.mark.gpu
+def test_destroyed_handles_throw(tmp_path):
+    """A destroyed problem or batch handle is refused, not used: the second destroy
+    throws, as does passing a problem handle where a batch is expected."""
+    script = tmp_path / "handles.mjs"
+    script.write_text(
+        "import { createRequire } from 'module';\n"
+        "const require = createRequire(import.meta.url);\n"
+        f"const A = require({json.dumps(os.path.join(JS, 'fdtd_hip.node'))});\n"
+        "const dims = { nx: 8, ny: 8, nz: 8, dx: 1e-3, dy: 1e-3, dz: 1e-3, dt: 1e-12,\n"
+        "               numberOfTimeSteps: 4, precision: 0, boundary: 0 };\n"
+        "const out = {};\n"
+        "const h = A.init(dims);\n"
+        "const m = A.multiCreate([h], null);\n"
+        "try { A.multiRun(h, 0, 1); } catch (e) { out.wrongKind = e.message; }\n"
+        "A.multiDestroy(m);\n"
+        "try { A.multiDestroy(m); } catch (e) { out.batchTwice = e.message; }\n"
+        "A.destroy(h);\n"
+        "try { A.destroy(h); } catch (e) { out.problemTwice = e.message; }\n"
+        "console.log(JSON.stringify(out));\n")
+    res = node([str(script)])
+    assert res.returncode == 0, res.stderr[-2000:]
+    out = json.loads(res.stdout.strip().splitlines()[-1])
+    assert "live batch handle" in out["wrongKind"]
+    assert "live batch handle" in out["batchTwice"]
+    assert "live problem handle" in out["problemTwice"]
