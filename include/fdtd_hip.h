@@ -109,6 +109,19 @@ int fdtd_hip_upload_svoltage(FdtdHipContext *ctx, int dir, const int *idx, int c
 int fdtd_hip_upload_scurrent(FdtdHipContext *ctx, const int *comp, const int *index,
                              const double *weight, int count, int negate);
 
+/* A point field probe (captureElectricFields / captureMagneticFields, sampling.js:
+ * 144-292), sampled at the end of every step: magnetic = 0 for E, 1 for H; component
+ * 0..2 = x, y, z, or 3 = the magnitude of the three; (i, j, k) = the 0-based cell
+ * (the reference's is-1, js-1, ks-1).  An E probe averages the two elements of its
+ * component back one cell along that component's own axis, an H probe the four back
+ * one cell along the two transverse axes, in the reference's order, in FP64.  E
+ * probes see the fields after the E update and source injection, H probes after the
+ * H update, as in the reference.  Upload before the first step; at most 64.  No
+ * counterpart in the reference's native ABI (its CUDA backend has no probes). */
+int fdtd_hip_upload_field_probe(FdtdHipContext *ctx, int magnetic, int component, int i, int j, int k);
+/* Probe traces of steps [tsStart, tsStart+count): out is [nProbes][count], FP64. */
+int fdtd_hip_read_probes(FdtdHipContext *ctx, int tsStart, int count, double *out);
+
 /* Huygens box and DFT kernels of the NTFF transform (farfield.js:28-78,90-248).
  * box = {li,lj,lk,ui,uj,uk} (1-based, as in initFarfield); phase = [nSteps][nFreq][4]
  * of cos(w dt (ts+1/2)), -sin(..), cos(w dt (ts+1)), -sin(..) as the host evaluates

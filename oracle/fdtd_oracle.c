@@ -13,6 +13,8 @@
  *   updateElectricCPML     js_fdtd/src/cpml.js:727-1003
  *   injectVoltageSources   js_fdtd/src/sources.js:271-288
  *   captureSampledVoltages js_fdtd/src/sampling.js:295-307
+ *   captureMagneticFields  js_fdtd/src/sampling.js:144-198 (point H probes)
+ *   captureElectricFields  js_fdtd/src/sampling.js:251-292 (point E probes)
  *   accumulateFarfieldDFT  js_fdtd/src/farfield.js:90-248
  *   step order             js_fdtd/src/fdtdSolver.js:84-140, fdtdWorker.js:57-125
  *
@@ -48,6 +50,7 @@ typedef struct {
 typedef struct { int dir, count; int *idx; double *coef, *vpe; int nsteps; } VSource;
 typedef struct { int dir, count; int *idx; double csvf; } SVoltage;
 typedef struct { int count, negate; int *comp, *index; double *weight; } SCurrent;
+typedef struct { int magnetic, comp, i, j, k; } Probe;   /* comp 0..2 = x,y,z; 3 = 'm' */
 
 typedef struct {
     int nx, ny, nz;
@@ -57,6 +60,7 @@ typedef struct {
     VSource vs[MAX_OBS]; int nvs;
     SVoltage sv[MAX_OBS]; int nsv;
     SCurrent si[MAX_OBS]; int nsi;
+    Probe pr[MAX_OBS]; int npr;
     /* NTFF */
     int nfreq, li, lj, lk, ui, uj, uk, ni, nj, nk, steps;
     double *phase;                 /* [steps][nfreq][4] */
@@ -406,6 +410,54 @@ void oracle_capture_currents(Oracle *o, double *out) {
     }
 }
 
+/* Point probe at the 0-based cell (i,j,k) = (is-1, js-1, ks-1).  Returns -1 when a
+ * probe slot is missing. */
+int oracle_add_probe(Oracle *o, int magnetic, int comp, int i, int j, int k) {
+    if (o->npr >= MAX_OBS) return -1;
+    Probe *p = &o->pr[o->npr++];
+    p->magnetic = magnetic; p->comp = comp; p->i = i; p->j = j; p->k = k;
+    return 0;
+}
+
+/* Element (i,j,k) of component c in the reference's flat layout. */
+static double at(Oracle *o, int c, int i, int j, int k) {
+    int d[3]; comp_dims(o, c, d);
+    return A(o, c)[((long)i * d[1] + j) * d[2] + k];
+}
+
+/* captureMagneticFields (sampling.js:144-198): 4-point averages. */
+static double h_avg(Oracle *o, int c, int is, int js, int ks) {
+    if (c == 0) return 0.25 * (at(o, HX, is, js-1, ks-1) + at(o, HX, is, js, ks-1) +
+                               at(o, HX, is, js-1, ks) + at(o, HX, is, js, ks));
+    if (c == 1) return 0.25 * (at(o, HY, is-1, js, ks-1) + at(o, HY, is, js, ks-1) +
+                               at(o, HY, is-1, js, ks) + at(o, HY, is, js, ks));
+    return 0.25 * (at(o, HZ, is-1, js-1, ks) + at(o, HZ, is, js-1, ks) +
+                   at(o, HZ, is-1, js, ks) + at(o, HZ, is, js, ks));
+}
+
+/* captureElectricFields (sampling.js:251-292): 2-point averages. */
+static double e_avg(Oracle *o, int c, int is, int js, int ks) {
+    if (c == 0) return 0.5 * (at(o, EX, is-1, js, ks) + at(o, EX, is, js, ks));
+    if (c == 1) return 0.5 * (at(o, EY, is, js-1, ks) + at(o, EY, is, js, ks));
+    return 0.5 * (at(o, EZ, is, js, ks-1) + at(o, EZ, is, js, ks));
+}
+
+/* The probes of one kind (magnetic or electric), in upload order, into out[p] for
+ * every probe p of that kind. */
+void oracle_capture_probes(Oracle *o, int magnetic, double *out) {
+    for (int n = 0; n < o->npr; n++) {
+        Probe *p = &o->pr[n];
+        if (p->magnetic != magnetic) continue;
+        double (*avg)(Oracle *, int, int, int, int) = magnetic ? h_avg : e_avg;
+        if (p->comp < 3) {
+            out[n] = avg(o, p->comp, p->i, p->j, p->k);
+        } else {
+            double x = avg(o, 0, p->i, p->j, p->k), y = avg(o, 1, p->i, p->j, p->k), z = avg(o, 2, p->i, p->j, p->k);
+            out[n] = sqrt(x*x + y*y + z*z);
+        }
+    }
+}
+
 /* ---------------------------------------------------------------------- NTFF -- */
 void oracle_ff_accumulate(Oracle *o, int ts) {
     if (o->nfreq <= 0) return;
@@ -495,20 +547,31 @@ void oracle_ff_accumulate(Oracle *o, int ts) {
 /* ------------------------------------------------------------------ the loop -- */
 /* Steps [ts0, ts0+count).  v_out / i_out: [count][n_sv] and [count][n_si] (may be
  * NULL).  Returns 0. */
-int oracle_run(Oracle *o, int ts0, int count, int boundary_mode, double *v_out, double *i_out) {
-    double vbuf[MAX_OBS], ibuf[MAX_OBS];
+/* As oracle_run, also filling p_out [count][n_probes] (may be NULL) with the point
+ * probes, H ones after the H update and E ones after the injection
+ * (fdtdSolver.js:96,116). */
+int oracle_run_probes(Oracle *o, int ts0, int count, int boundary_mode, double *v_out, double *i_out,
+                      double *p_out) {
+    double vbuf[MAX_OBS], ibuf[MAX_OBS], pbuf[MAX_OBS];
     for (int s = 0; s < count; s++) {
         int ts = ts0 + s;
         oracle_update_h(o);
         oracle_update_h_cpml(o);
+        oracle_capture_probes(o, 1, pbuf);
         oracle_capture_currents(o, ibuf);
         oracle_update_e(o, boundary_mode);
         oracle_update_e_cpml(o);
         oracle_inject(o, ts);
+        oracle_capture_probes(o, 0, pbuf);
         oracle_capture_voltages(o, vbuf);
         oracle_ff_accumulate(o, ts);
         if (v_out) for (int n = 0; n < o->nsv; n++) v_out[(long)s * o->nsv + n] = vbuf[n];
         if (i_out) for (int n = 0; n < o->nsi; n++) i_out[(long)s * o->nsi + n] = ibuf[n];
+        if (p_out) for (int n = 0; n < o->npr; n++) p_out[(long)s * o->npr + n] = pbuf[n];
     }
     return 0;
+}
+
+int oracle_run(Oracle *o, int ts0, int count, int boundary_mode, double *v_out, double *i_out) {
+    return oracle_run_probes(o, ts0, count, boundary_mode, v_out, i_out, NULL);
 }

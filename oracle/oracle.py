@@ -63,6 +63,9 @@ def lib():
         L.oracle_capture_currents.argtypes = [P, D]
         L.oracle_ff_accumulate.argtypes = [P, I]
         L.oracle_run.argtypes = [P, I, I, I, D, D]
+        L.oracle_run_probes.argtypes = [P, I, I, I, D, D, D]
+        L.oracle_add_probe.argtypes = [P, I, I, I, I, I]
+        L.oracle_capture_probes.argtypes = [P, I, D]
         _lib = L
     return _lib
 
@@ -122,6 +125,10 @@ class OracleSim:
                             [ff["ni_ff"] * ff["nj_ff"]] * 8
         else:
             self.nfreq = 0
+        self.probe_list = problem.header.get("probes", [])
+        for pr in self.probe_list:
+            if self.L.oracle_add_probe(self.h, pr["magnetic"], pr["component"], pr["i"], pr["j"], pr["k"]):
+                raise ValueError("too many field probes for the oracle")
         self.L.oracle_set_threads(self.h, threads)
         self.n_sv = len(problem.header["svoltages"])
         self.n_si = len(problem.header["scurrents"])
@@ -184,6 +191,22 @@ class OracleSim:
 
     def ff_accumulate(self, ts: int):
         self.L.oracle_ff_accumulate(self.h, ts)
+
+    def run_probes(self, ts0: int, count: int):
+        """Run steps [ts0, ts0+count); returns (V[count, n_sv], I[count, n_si],
+        probes[count, n_probes])."""
+        n = len(self.probe_list)
+        v = np.zeros((count, max(self.n_sv, 1)))
+        i = np.zeros((count, max(self.n_si, 1)))
+        p = np.zeros((count, max(n, 1)))
+        self.L.oracle_run_probes(self.h, ts0, count, self.boundary_mode, _dp(v), _dp(i), _dp(p))
+        return v[:, : self.n_sv], i[:, : self.n_si], p[:, :n]
+
+    def capture_probes(self, magnetic: bool) -> np.ndarray:
+        """Probe values of one kind from the current fields (others left at zero)."""
+        out = np.zeros(max(len(self.probe_list), 1))
+        self.L.oracle_capture_probes(self.h, 1 if magnetic else 0, _dp(out))
+        return out[: len(self.probe_list)]
 
     def run(self, ts0: int, count: int):
         """Run steps [ts0, ts0+count); returns (V[count, n_sv], I[count, n_si])."""

@@ -80,6 +80,7 @@ struct FdtdHipContext {
 
   std::vector<void *> small_allocs;   // port tables, traces
   PortState ports{};
+  std::vector<int> probe_host;        // [npr][PROBE_INTS], uploaded with the traces
   int traces_ready = 0;
 
   int nff = 0;
@@ -267,7 +268,14 @@ int ensure_traces(FdtdHipContext *c) {
   const int steps = c->d.numberOfTimeSteps;
   std::vector<double> zv((size_t)std::max(1, c->ports.nsv) * steps, 0.0);
   std::vector<double> zi((size_t)std::max(1, c->ports.nsi) * steps, 0.0);
-  if (upload_small(c, zv, &c->ports.vtr) || upload_small(c, zi, &c->ports.itr)) return FDTD_HIP_ERR;
+  std::vector<double> zp((size_t)std::max(1, c->ports.npr) * steps, 0.0);
+  std::vector<int> pr(c->probe_host);
+  if (pr.empty()) pr.assign(PROBE_INTS, 0);
+  int *prd = nullptr;
+  if (upload_small(c, zv, &c->ports.vtr) || upload_small(c, zi, &c->ports.itr) ||
+      upload_small(c, zp, &c->ports.prtr) || upload_small(c, pr, &prd))
+    return FDTD_HIP_ERR;
+  c->ports.pr = prd;
   c->ports.steps = steps;
   c->ports.dt = c->d.dt;
   c->traces_ready = 1;
@@ -578,7 +586,7 @@ int launch_step(FdtdHipContext *c, int ts, const int *tsp = nullptr) {
     hipLaunchKernelGGL(k_cpml_e<T>, dim3((unsigned)(((long long)c->cpml_e.start[c->cpml_e.nbox] + 255) / 256)), block, 0,
                        c->stream, g, fs, c->cpml_e);
   if (t) HIP_TRY(hipEventRecord(c->ev[4], c->stream));
-  if (c->ports.nvs + c->ports.nsv + c->ports.nsi > 0) {
+  if (c->ports.nvs + c->ports.nsv + c->ports.nsi + c->ports.npr > 0) {
     PortState ps = c->ports;
     ps.ndft = c->ndft; ps.w = c->dft_w; ps.dft = c->dft_acc;
     hipLaunchKernelGGL(k_port_epilogue<T>, dim3(1), block, 0, c->stream, g, ps, tsp, ts);
@@ -952,6 +960,51 @@ int fdtd_hip_read_traces(FdtdHipContext *c, int tsStart, int count, double *svOu
   return FDTD_HIP_OK;
 }
 
+int fdtd_hip_upload_field_probe(FdtdHipContext *c, int magnetic, int component, int i, int j, int k) {
+  if (!c) return fail("upload_field_probe: null context");
+  drop_graph(c);
+  if (magnetic != 0 && magnetic != 1) return fail("upload_field_probe: magnetic must be 0 or 1");
+  if (component < 0 || component > 3) return fail("upload_field_probe: component %d out of range (0..3)", component);
+  if (c->traces_ready) return fail("upload_field_probe: probes must be uploaded before the first step");
+  if (c->ports.npr >= MAX_FIELD_PROBES) return fail("upload_field_probe: at most %d probes", MAX_FIELD_PROBES);
+  // Averaged elements (sampling.js:144-292): E along its own axis back one cell; H over
+  // the two transverse axes back one cell, in the reference's summation order.
+  static const int kE[3][2][3] = {{{-1, 0, 0}, {0, 0, 0}}, {{0, -1, 0}, {0, 0, 0}}, {{0, 0, -1}, {0, 0, 0}}};
+  static const int kH[3][4][3] = {{{0, -1, -1}, {0, 0, -1}, {0, -1, 0}, {0, 0, 0}},
+                                  {{-1, 0, -1}, {0, 0, -1}, {-1, 0, 0}, {0, 0, 0}},
+                                  {{-1, -1, 0}, {0, -1, 0}, {-1, 0, 0}, {0, 0, 0}}};
+  int d[PROBE_INTS] = {magnetic, component, magnetic ? 4 : 2, 0};
+  for (int cc = 0; cc < 3; ++cc) {
+    if (component < 3 && cc != component) continue;
+    const int comp = (magnetic ? 0 : 3) + cc;
+    int dims[3];
+    c->comp_dims(comp, dims);
+    for (int t = 0; t < (magnetic ? 4 : 2); ++t) {
+      const int *o = magnetic ? kH[cc][t] : kE[cc][t];
+      const int ii = i + o[0], jj = j + o[1], kk = k + o[2];
+      if (ii < 0 || ii >= dims[0] || jj < 0 || jj >= dims[1] || kk < 0 || kk >= dims[2])
+        return fail("upload_field_probe: %c%c(%d,%d,%d) outside its %dx%dx%d array", magnetic ? 'H' : 'E',
+                    "xyz"[cc], ii, jj, kk, dims[0], dims[1], dims[2]);
+      d[4 + 4 * cc + t] = (int)c->to_dev(comp, ((long long)ii * dims[1] + jj) * dims[2] + kk);
+    }
+  }
+  c->probe_host.insert(c->probe_host.end(), d, d + PROBE_INTS);
+  c->ports.npr++;
+  return FDTD_HIP_OK;
+}
+
+int fdtd_hip_read_probes(FdtdHipContext *c, int tsStart, int count, double *out) {
+  if (!c || !out) return fail("read_probes: null argument");
+  if (tsStart < 0 || count < 0 || tsStart + count > c->d.numberOfTimeSteps)
+    return fail("read_probes: steps [%d,%d) outside [0,%d)", tsStart, tsStart + count, c->d.numberOfTimeSteps);
+  if (!c->traces_ready) return fail("read_probes: nothing has run yet");
+  for (int p = 0; p < c->ports.npr; ++p)
+    HIP_TRY(hipMemcpyAsync(out + (size_t)p * count, c->ports.prtr + (size_t)p * c->ports.steps + tsStart,
+                           count * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return FDTD_HIP_OK;
+}
+
 int fdtd_hip_run_batch(FdtdHipContext *c, int tsStart, int count, double *svOut, double *siOut) {
   if (fdtd_hip_run_batch_async(c, tsStart, count)) return FDTD_HIP_ERR;
   return fdtd_hip_read_traces(c, tsStart, count, svOut, siOut);
@@ -1201,6 +1254,7 @@ static int reset_state(FdtdHipContext *c) {
   if (c->traces_ready) {
     HIP_TRY(hipMemsetAsync(c->ports.vtr, 0, sizeof(double) * std::max(1, c->ports.nsv) * c->ports.steps, c->stream));
     HIP_TRY(hipMemsetAsync(c->ports.itr, 0, sizeof(double) * std::max(1, c->ports.nsi) * c->ports.steps, c->stream));
+    HIP_TRY(hipMemsetAsync(c->ports.prtr, 0, sizeof(double) * std::max(1, c->ports.npr) * c->ports.steps, c->stream));
   }
   HIP_TRY(hipStreamSynchronize(c->stream));
   return FDTD_HIP_OK;
@@ -1397,7 +1451,7 @@ int multi_prepare(FdtdHipMulti *m) {
       if (m->ntff && m->ntff != mode) return fail("multi: variants mix blocked and direct NTFF accumulation");
       m->ntff = mode;
     }
-    if (c->ports.nvs + c->ports.nsv + c->ports.nsi > 0) m->ports = true;
+    if (c->ports.nvs + c->ports.nsv + c->ports.nsi + c->ports.npr > 0) m->ports = true;
   }
 
   const int n = (int)m->ctx.size();
@@ -1416,7 +1470,7 @@ int multi_prepare(FdtdHipMulti *m) {
     d.be = c->cpml_e;
     d.ps = c->ports;
     d.ps.ndft = c->ndft; d.ps.w = c->dft_w; d.ps.dft = c->dft_acc;
-    d.has_ports = c->ports.nvs + c->ports.nsv + c->ports.nsi > 0;
+    d.has_ports = c->ports.nvs + c->ports.nsv + c->ports.nsi + c->ports.npr > 0;
     d.ff = c->ff;
     if (ntff_blocked(c)) d.nb = ntff_block<T>(c);
     const long th = (long)d.mh.ncol * d.mh.nchunk, te = (long)d.me.ncol * d.me.nchunk;

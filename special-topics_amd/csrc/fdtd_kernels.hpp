@@ -611,12 +611,20 @@ struct VSourceDev { int dir, count; const int *idx; const double *coef, *wave; }
 struct SVoltageDev { int dir, count; const int *idx; double csvf; };
 struct SCurrentDev { int count, negate; const int *comp, *idx; const double *w; };
 
+// A point field probe: magnetic flag, component (0..2, or 3 = magnitude), then the
+// device offsets of the 2 (E) or 4 (H) averaged elements of each of Fx, Fy, Fz.
+constexpr int PROBE_INTS = 16;
+constexpr int MAX_FIELD_PROBES = 64;
+
 struct PortState {
   int nvs, nsv, nsi;
   VSourceDev vs[MAX_PORT_OBJS];
   SVoltageDev sv[MAX_PORT_OBJS];
   SCurrentDev si[MAX_PORT_OBJS];
   double *vtr, *itr;      // [nsv][steps], [nsi][steps]
+  int npr;                // point field probes
+  const int *pr;          // [npr][PROBE_INTS]
+  double *prtr;           // [npr][steps]
   int steps;
   int ndft;               // running DFT frequencies (0 = off)
   const double *w;        // angular frequencies
@@ -637,8 +645,27 @@ __device__ __forceinline__ double ordered_wave_sum(double acc, double mine, int 
 // of the first step of a replayed graph plus the kernel's offset within the graph.
 __device__ __forceinline__ int step_of(const int *tsp, int off) { return off + (tsp ? *tsp : 0); }
 
-// Source injection, then port voltages and currents, then their running DFTs, for
-// step ts; run by one whole block.  The injection precedes the captures as in
+// Value of a point field probe (captureElectricFields / captureMagneticFields,
+// js_fdtd/src/sampling.js:144-292): the FP64 average of 2 E or 4 H elements, summed
+// left to right as the reference does, or the magnitude of the three averages.
+template <typename T>
+__device__ __forceinline__ double probe_value(const Grid<T> &g, const int *d) {
+#pragma clang fp contract(off)
+  const bool mag = d[0] != 0;
+  const T *F[3] = {mag ? g.Hx : g.Ex, mag ? g.Hy : g.Ey, mag ? g.Hz : g.Ez};
+  auto avg = [&](int c) -> double {
+    const int *ix = d + 4 + 4 * c;
+    const T *f = F[c];
+    if (!mag) return 0.5 * ((double)f[ix[0]] + (double)f[ix[1]]);
+    return 0.25 * ((((double)f[ix[0]] + (double)f[ix[1]]) + (double)f[ix[2]]) + (double)f[ix[3]]);
+  };
+  if (d[1] < 3) return avg(d[1]);
+  const double x = avg(0), y = avg(1), z = avg(2);
+  return sqrt(x * x + y * y + z * z);
+}
+
+// Source injection, then port voltages and currents and the point field probes, then
+// the running DFTs of the port quantities, for step ts; run by one whole block.  The injection precedes the captures as in
 // fdtdSolver.js:111-117.
 template <typename T>
 __device__ __forceinline__ void port_epilogue_body(const Grid<T> &g, const PortState &ps, int ts) {
@@ -681,6 +708,10 @@ __device__ __forceinline__ void port_epilogue_body(const Grid<T> &g, const PortS
       if (lane == 0) { ps.itr[(long)(o - ps.nsv) * ps.steps + ts] = acc; s_val[o] = acc; }
     }
   }
+  // Probes read the fields after the E update and the injection; H does not change
+  // after the H phase, so H probes see the values captureMagneticFields sees.
+  for (int p = tid; p < ps.npr; p += blockDim.x)
+    ps.prtr[(long)p * ps.steps + ts] = probe_value<T>(g, ps.pr + (long)p * PROBE_INTS);
   if (ps.ndft == 0) return;
   __syncthreads();
   const int ntr = ps.nsv + ps.nsi;

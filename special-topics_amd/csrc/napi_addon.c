@@ -27,6 +27,7 @@
  *   setTiming(h, bool); kernelTimes(h) -> {ms:[6], launches:[6]}
  *   destroy(h)
  *   readTraces(h, tsStart, count, Float64Array|null sv, Float64Array|null si)
+ *   uploadFieldProbe(h, magnetic, component, i, j, k); readProbes(h, tsStart, count, Float64Array)
  *   multiCreate([h, ...], tuning|null) -> batch handle   (fdtd_hip_multi_*: many
  *   multiRun(m, tsStart, count); multiSynchronize(m)       problems, one launch per
  *   multiAutotune(m, stepsPerCandidate) -> tuning          phase)
@@ -554,6 +555,31 @@ static napi_value ReadTraces(napi_env env, napi_callback_info info) {
   return undefined(env);
 }
 
+static napi_value UploadFieldProbe(napi_env env, napi_callback_info info) {
+  napi_value argv[6];
+  int a[5];
+  if (!get_args(env, info, 6, argv)) return NULL;
+  FdtdHipContext *c = handle(env, argv[0]);
+  if (!c) return NULL;
+  for (int q = 0; q < 5; ++q)
+    if (!get_int(env, argv[q + 1], &a[q])) return NULL;
+  if (fdtd_hip_upload_field_probe(c, a[0], a[1], a[2], a[3], a[4]) != FDTD_HIP_OK)
+    return throw_lib(env, "uploadFieldProbe");
+  return undefined(env);
+}
+
+static napi_value ReadProbes(napi_env env, napi_callback_info info) {
+  napi_value argv[4];
+  int ts, count, ok;
+  if (!get_args(env, info, 4, argv)) return NULL;
+  FdtdHipContext *c = handle(env, argv[0]);
+  if (!c || !get_int(env, argv[1], &ts) || !get_int(env, argv[2], &count)) return NULL;
+  double *out = typed(env, argv[3], napi_float64_array, NULL, 0, &ok);
+  if (!ok) return NULL;
+  if (fdtd_hip_read_probes(c, ts, count, out) != FDTD_HIP_OK) return throw_lib(env, "readProbes");
+  return undefined(env);
+}
+
 static FdtdHipMulti *multi_handle(napi_env env, napi_value v) {
   HandleBox *b = box_of(env, v, 2);
   return b ? (FdtdHipMulti *)b->p : NULL;
@@ -668,6 +694,8 @@ static napi_value Register(napi_env env, napi_value exports) {
     {"kernelTimes", NULL, KernelTimes, NULL, NULL, NULL, napi_enumerable, NULL},
     {"destroy", NULL, Destroy, NULL, NULL, NULL, napi_enumerable, NULL},
     {"readTraces", NULL, ReadTraces, NULL, NULL, NULL, napi_enumerable, NULL},
+    {"uploadFieldProbe", NULL, UploadFieldProbe, NULL, NULL, NULL, napi_enumerable, NULL},
+    {"readProbes", NULL, ReadProbes, NULL, NULL, NULL, napi_enumerable, NULL},
     {"multiCreate", NULL, MultiCreate, NULL, NULL, NULL, napi_enumerable, NULL},
     {"multiRun", NULL, MultiRun, NULL, NULL, NULL, napi_enumerable, NULL},
     {"multiSynchronize", NULL, MultiSynchronize, NULL, NULL, NULL, napi_enumerable, NULL},

@@ -47,7 +47,8 @@ class FdtdHipKernelTimes(C.Structure):
 EXPORTS = [
     "fdtd_hip_last_error", "fdtd_hip_device_count", "fdtd_hip_init", "fdtd_hip_upload_coeff",
     "fdtd_hip_upload_coeff_f32", "fdtd_hip_upload_cpml", "fdtd_hip_upload_vsource",
-    "fdtd_hip_upload_svoltage", "fdtd_hip_upload_scurrent", "fdtd_hip_upload_farfield",
+    "fdtd_hip_upload_svoltage", "fdtd_hip_upload_scurrent", "fdtd_hip_upload_field_probe",
+    "fdtd_hip_read_probes", "fdtd_hip_upload_farfield",
     "fdtd_hip_upload_port_dft", "fdtd_hip_run_batch", "fdtd_hip_run_batch_async", "fdtd_hip_read_traces",
     "fdtd_hip_query", "fdtd_hip_read_field", "fdtd_hip_write_field",
     "fdtd_hip_read_psi", "fdtd_hip_write_psi", "fdtd_hip_read_farfield", "fdtd_hip_read_port_dft", "fdtd_hip_radiated_power", "fdtd_hip_directivity",
@@ -96,6 +97,8 @@ def lib():
         L.fdtd_hip_upload_vsource.argtypes = [P, I, IP, D, I, D, I]
         L.fdtd_hip_upload_svoltage.argtypes = [P, I, IP, I, C.c_double]
         L.fdtd_hip_upload_scurrent.argtypes = [P, IP, IP, D, I, I]
+        L.fdtd_hip_upload_field_probe.argtypes = [P, I, I, I, I, I]
+        L.fdtd_hip_read_probes.argtypes = [P, I, I, D]
         L.fdtd_hip_upload_farfield.argtypes = [P, I, IP, D, I]
         L.fdtd_hip_upload_port_dft.argtypes = [P, I, D]
         L.fdtd_hip_run_batch.argtypes = [P, I, I, D, D]

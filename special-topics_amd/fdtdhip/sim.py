@@ -91,6 +91,10 @@ class HipSim:
             w = np.ascontiguousarray(P.arrays[f"si/{n}/weight"], dtype=np.float64)
             native.check(L.fdtd_hip_upload_scurrent(h, _i(comp), _i(index), _d(w), comp.size, int(si["negate"])),
                          "upload_scurrent")
+        self.probe_list = P.header.get("probes", [])
+        for pr in self.probe_list:
+            native.check(L.fdtd_hip_upload_field_probe(h, pr["magnetic"], pr["component"], pr["i"], pr["j"], pr["k"]),
+                         "upload_field_probe")
         ff = P.header["ff"]
         self.nfreq = 0
         if farfield and ff.get("nFreq", 0) > 0 and "ff/phase" in P.arrays:
@@ -134,6 +138,13 @@ class HipSim:
         i = np.zeros((max(self.n_si, 1), count))
         native.check(self.L.fdtd_hip_read_traces(self.h, ts0, count, _d(v), _d(i)), "read_traces")
         return v[: self.n_sv], i[: self.n_si]
+
+    def probes(self, ts0: int, count: int) -> np.ndarray:
+        """Point field probe traces of steps [ts0, ts0+count): [n_probes, count]."""
+        n = len(self.probe_list)
+        out = np.zeros((max(n, 1), count))
+        native.check(self.L.fdtd_hip_read_probes(self.h, ts0, count, _d(out)), "read_probes")
+        return out[:n]
 
     def idle(self) -> bool:
         b = C.c_int(0)

@@ -124,6 +124,7 @@ export function uploadProblem(A, problem, opts) {
       A.uploadSCurrent(h, asInt32(arrays[`si/${n}/comp`]), asInt32(arrays[`si/${n}/index`]),
                        asFloat64(arrays[`si/${n}/weight`]), si.negate ? 1 : 0);
     });
+    for (const pr of header.probes) A.uploadFieldProbe(h, pr.magnetic, pr.component, pr.i, pr.j, pr.k);
     if (header.ff.nFreq > 0) {
       const b = header.ff;
       A.uploadFarfield(h, b.nFreq, Int32Array.of(b.li, b.lj, b.lk, b.ui, b.uj, b.uk), arrays['ff/phase']);
@@ -207,12 +208,18 @@ function collectResults(A, h, problem, opts, header, ff) {
   }
 }
 
-function checkProbes(problem) {
+// The point field probes in upload order (E, then H), as dump.mjs lists them.
+function probeList(problem) {
   const { samplers } = problem;
-  const probes = ((samplers && samplers.sampledEFields) || []).length + ((samplers && samplers.sampledHFields) || []).length;
-  if (probes > 0) {
-    throw new Error('hip backend: point field probes (sampledEFields / sampledHFields) are not supported');
-  }
+  return [...((samplers && samplers.sampledEFields) || []), ...((samplers && samplers.sampledHFields) || [])];
+}
+
+// Copy probe traces of steps [start, start+count) into each probe's sampled_value.
+function readProbes(A, h, probes, start, count, buf) {
+  if (probes.length === 0) return;
+  const p = buf.subarray(0, probes.length * count);
+  A.readProbes(h, start, count, p);
+  probes.forEach((o, n) => o.sampled_value.set(p.subarray(n * count, (n + 1) * count), start));
 }
 
 function requireAddon() {
@@ -252,7 +259,7 @@ const backend = {
     const total = grid.numberOfTimeSteps;
     const sampledVoltages = (samplers && samplers.sampledVoltages) || [];
     const sampledCurrents = (samplers && samplers.sampledCurrents) || [];
-    checkProbes(problem);
+    const probes = probeList(problem);
     const tSetup = Date.now();
     const { h, header, ff } = uploadProblem(A, problem, opts);
     try {
@@ -262,6 +269,7 @@ const backend = {
       const nsv = sampledVoltages.length, nsi = sampledCurrents.length;
       const vBuf = new Float64Array(Math.max(nsv, 1) * opts.batchSize);
       const iBuf = new Float64Array(Math.max(nsi, 1) * opts.batchSize);
+      const pBuf = new Float64Array(Math.max(probes.length, 1) * opts.batchSize);
       const startTime = Date.now();
       for (let start = 0; start < total; start += opts.batchSize) {
         const count = Math.min(opts.batchSize, total - start);
@@ -271,6 +279,7 @@ const backend = {
         // Traces come back observer-major: [observer][step of the batch].
         for (let o = 0; o < nsv; o++) sampledVoltages[o].sampled_value.set(v.subarray(o * count, (o + 1) * count), start);
         for (let o = 0; o < nsi; o++) sampledCurrents[o].sampled_value.set(c.subarray(o * count, (o + 1) * count), start);
+        readProbes(A, h, probes, start, count, pBuf);
         yield snapshot(start + count - 1, total, startTime, sampledVoltages, grid.dt);
         await new Promise((r) => setTimeout(r, 0));
       }
@@ -293,7 +302,6 @@ const backend = {
     const opts = problems.map((p) => resolveOptions(p.options));
     const total = problems[0].grid.numberOfTimeSteps;
     problems.forEach((p, n) => {
-      checkProbes(p);
       if (opts[n].precision !== opts[0].precision) throw new Error('hip backend: runMany needs one precision');
       if (p.grid.numberOfTimeSteps !== total) throw new Error('hip backend: runMany needs one step count');
     });
@@ -319,6 +327,8 @@ const backend = {
         A.readTraces(h, 0, total, sv.length ? v : null, si.length ? c : null);
         sv.forEach((o, k) => o.sampled_value.set(v.subarray(k * total, (k + 1) * total)));
         si.forEach((o, k) => o.sampled_value.set(c.subarray(k * total, (k + 1) * total)));
+        const probes = probeList(p);
+        readProbes(A, h, probes, 0, total, new Float64Array(Math.max(probes.length, 1) * total));
         collectResults(A, h, p, opts[n], header, ff);
       });
       lastRun.run_s = (Date.now() - startTime) / 1000;

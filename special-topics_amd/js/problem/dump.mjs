@@ -35,7 +35,7 @@ export function problemArrays(problem, opts) {
       numberOfTimeSteps: grid.numberOfTimeSteps,
     },
     coeff_dtype: o.fp32 ? 'f32' : 'f64',
-    cpml: [], vsources: [], svoltages: [], scurrents: [], ff: { nFreq: 0 },
+    cpml: [], vsources: [], svoltages: [], scurrents: [], probes: [], ff: { nFreq: 0 },
     ports: problem.ports || [], dft: problem.dft || null, meta: problem.meta || {},
   };
   if (problem.materialGrid) arrays.mat = problem.materialGrid;
@@ -74,6 +74,15 @@ export function problemArrays(problem, opts) {
     arrays[`si/${n}/comp`] = t.comp;
     arrays[`si/${n}/index`] = t.index;
     arrays[`si/${n}/weight`] = t.weight;
+  });
+  // Point field probes, E then H, each as the 0-based cell (is-1, js-1, ks-1) and its
+  // component (0..2 = x, y, z; 3 = magnitude 'm').
+  [[samplers.sampledEFields || [], 0], [samplers.sampledHFields || [], 1]].forEach(([list, magnetic]) => {
+    for (const pr of list) {
+      const component = 'xyzm'.indexOf(pr.component);
+      if (component < 0) throw new Error(`field probe component "${pr.component}" is not x, y, z or m`);
+      header.probes.push({ magnetic, component, i: pr.is - 1, j: pr.js - 1, k: pr.ks - 1, label: pr.label || null });
+    }
   });
   if (ff && ff.nFreq > 0) {
     header.ff = {

@@ -37,6 +37,8 @@ async function main() {
     snapshots: { count: snaps.length, first: snaps[0], last: snaps[snaps.length - 1] },
     v: sim.samplers.sampledVoltages.map((o) => b64(o.sampled_value)),
     i: sim.samplers.sampledCurrents.map((o) => b64(o.sampled_value)),
+    probes: [...(sim.samplers.sampledEFields || []), ...(sim.samplers.sampledHFields || [])]
+      .map((o) => b64(o.sampled_value)),
     fields: {}, ff: {},
   };
   for (const c of FIELD_ORDER) rec.fields[c] = sha(sim.grid.fields[c]);

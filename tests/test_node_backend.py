@@ -24,6 +24,7 @@ JS = os.path.join(REPO, "special-topics_amd", "js")
 DRIVER = os.path.join(REPO, "tests", "js", "backend_driver.mjs")
 SCEN = {
     "golden-small": os.path.join(REPO, "tests", "golden", "scenarios", "golden-small.json"),
+    "golden-probes": os.path.join(REPO, "tests", "golden", "scenarios", "golden-probes.json"),
     "ifa-matlab-7000": os.path.join(REPO, "special-topics_amd", "scenarios", "ifa-matlab-7000.json"),
     "ifa-config1-7000": os.path.join(REPO, "special-topics_amd", "scenarios", "ifa-config1-7000.json"),
 }
