@@ -187,14 +187,12 @@ __device__ __forceinline__ T cpml_step(const FaceSet<T> &fs, int s, int ci, T va
 }
 
 // One component's CPML terms on the value v: faces of axis A1 (n then p) then of axis
-// A2, each with its curl difference computed on demand; *hit tells whether any face
-// applied.
+// A2, each with its curl difference computed on demand.
 template <typename T, int A1, int S1, int A2, int S2, bool MAG, typename D1, typename D2>
 __device__ __forceinline__ T cpml_terms(const FaceSet<T> &fs, T v, int i, int j, int k, int d1, int d2,
-                                        D1 diff1, D2 diff2, bool *hit) {
+                                        D1 diff1, D2 diff2) {
   const int a0 = cpml_slot<T, A1, MAG>(fs, 0, i, j, k), a1 = cpml_slot<T, A1, MAG>(fs, 1, i, j, k);
   const int b0 = cpml_slot<T, A2, MAG>(fs, 0, i, j, k), b1 = cpml_slot<T, A2, MAG>(fs, 1, i, j, k);
-  *hit = a0 >= 0 || a1 >= 0 || b0 >= 0 || b1 >= 0;
   if (a0 >= 0 || a1 >= 0) {
     const T d = diff1();
     if (a0 >= 0) v = cpml_step<T, A1, S1, MAG>(fs, 0, a0, v, d, i, j, k, d1, d2);
@@ -215,8 +213,7 @@ __device__ __forceinline__ void cpml_component(const FaceSet<T> &fs, T *__restri
   const int a0 = cpml_slot<T, A1, MAG>(fs, 0, i, j, k), a1 = cpml_slot<T, A1, MAG>(fs, 1, i, j, k);
   const int b0 = cpml_slot<T, A2, MAG>(fs, 0, i, j, k), b1 = cpml_slot<T, A2, MAG>(fs, 1, i, j, k);
   if (a0 < 0 && a1 < 0 && b0 < 0 && b1 < 0) return;
-  bool hit;
-  F[c] = cpml_terms<T, A1, S1, A2, S2, MAG>(fs, F[c], i, j, k, d1, d2, diff1, diff2, &hit);
+  F[c] = cpml_terms<T, A1, S1, A2, S2, MAG>(fs, F[c], i, j, k, d1, d2, diff1, diff2);
 }
 
 // Whether coordinate v along AXIS lies in a slab of that axis's faces (MAG: H planes,
@@ -342,16 +339,15 @@ __device__ __forceinline__ void update_h_body(
       for (int e = 0; e < W; ++e) {
         const int k = k0 + e;
         if (k > nz || !fused_cpml_cell<T, FC, true>(fs, i, j, k)) continue;
-        bool hit;
         if (j < ny && k < nz)                              // Hx: y then z faces
           hx[e] = cpml_terms<T, 1, 0, 2, 0, true>(fs, hx[e], i, j, k, ny, nz,
-              [&] { return dy_hx[e]; }, [&] { return dz_hx[e]; }, &hit);
+              [&] { return dy_hx[e]; }, [&] { return dz_hx[e]; });
         if (inner && k < nz)                               // Hy: x then z faces
           hy[e] = cpml_terms<T, 0, 0, 2, 1, true>(fs, hy[e], i, j, k, ny + 1, nz,
-              [&] { return dx_hy[e]; }, [&] { return dz_hy[e]; }, &hit);
+              [&] { return dx_hy[e]; }, [&] { return dz_hy[e]; });
         if (inner && j < ny)                               // Hz: x then y faces
           hz[e] = cpml_terms<T, 0, 1, 1, 1, true>(fs, hz[e], i, j, k, ny, nz + 1,
-              [&] { return dx_hz[e]; }, [&] { return dy_hz[e]; }, &hit);
+              [&] { return dx_hz[e]; }, [&] { return dy_hz[e]; });
       }
     }
     stv<T, W>(Hx + c, hx);
@@ -448,16 +444,15 @@ __device__ __forceinline__ void update_e_body(
       for (int e = 0; e < W; ++e) {
         const int k = k0 + e;
         if (k > nz || !fused_cpml_cell<T, FC, false>(fs, i, j, k)) continue;
-        bool hit;
         if (i < nx)                                        // Ex: y then z faces
           ex[e] = cpml_terms<T, 1, 3, 2, 2, false>(fs, ex[e], i, j, k, ny + 1, nz + 1,
-              [&] { return dy_ex[e]; }, [&] { return dz_ex[e]; }, &hit);
+              [&] { return dy_ex[e]; }, [&] { return dz_ex[e]; });
         if (j < ny)                                        // Ey: x then z faces
           ey[e] = cpml_terms<T, 0, 2, 2, 3, false>(fs, ey[e], i, j, k, ny, nz + 1,
-              [&] { return dx_ey[e]; }, [&] { return dz_ey[e]; }, &hit);
+              [&] { return dx_ey[e]; }, [&] { return dz_ey[e]; });
         if (k < nz)                                        // Ez: x then y faces
           ez[e] = cpml_terms<T, 0, 3, 1, 2, false>(fs, ez[e], i, j, k, ny + 1, nz,
-              [&] { return dx_ez[e]; }, [&] { return dy_ez[e]; }, &hit);
+              [&] { return dx_ez[e]; }, [&] { return dy_ez[e]; });
       }
     }
     if (i < nx) stv<T, W>(Ex + c, ex);
@@ -492,13 +487,12 @@ __device__ __forceinline__ void update_e_body(
       for (int e = 0; e < W; ++e) {
         const int k = k0 + e, ce = cg + e;
         if (k > nz || !fused_cpml_cell<T, FC, false>(fs, nx, j, k)) continue;
-        bool hit;
         if (j < ny)                                        // Ey(nx): x then z faces
           Ey[ce] = cpml_terms<T, 0, 2, 2, 3, false>(fs, Ey[ce], nx, j, k, ny, nz + 1,
-              [&] { return Hz[ce] - Hz[ce - SI]; }, [&] { return Hx[ce] - Hx[ce - 1]; }, &hit);
+              [&] { return Hz[ce] - Hz[ce - SI]; }, [&] { return Hx[ce] - Hx[ce - 1]; });
         if (k < nz)                                        // Ez(nx): x then y faces
           Ez[ce] = cpml_terms<T, 0, 3, 1, 2, false>(fs, Ez[ce], nx, j, k, ny + 1, nz,
-              [&] { return Hy[ce] - Hy[ce - SI]; }, [&] { return Hx[ce] - Hx[ce - SJ]; }, &hit);
+              [&] { return Hy[ce] - Hy[ce - SI]; }, [&] { return Hx[ce] - Hx[ce - SJ]; });
       }
     }
     hz_m = hz_i;

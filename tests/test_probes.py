@@ -142,7 +142,6 @@ def test_batched_probes_match_single(get_problem):
 
 @pytest.mark.gpu
 def test_out_of_range_probe_is_refused(get_problem):
-    import ctypes as C
     from fdtdhip import native
     from fdtdhip.sim import HipSim
     sim = HipSim(get_problem("golden-small"))
