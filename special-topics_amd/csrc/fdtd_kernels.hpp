@@ -82,6 +82,26 @@ __device__ __forceinline__ Vec<T, W> ldv(const T *__restrict__ p) { return *(con
 template <typename T, int W>
 __device__ __forceinline__ void stv(T *__restrict__ p, const Vec<T, W> &v) { *(Vec<T, W> *)p = v; }
 
+// Non-temporal store of W cells (the own field of a stencil: written once, read
+// next by the other phase from HBM).
+template <typename T, int W>
+__device__ __forceinline__ void stv_stream(T *__restrict__ p, const Vec<T, W> &v) {
+  if (W * sizeof(T) == 16) {
+    typedef float f4 __attribute__((ext_vector_type(4)));
+    f4 x;
+    __builtin_memcpy(&x, &v, 16);
+    __builtin_nontemporal_store(x, (f4 *)p);
+  } else if (W * sizeof(T) == 8) {
+    typedef float f2 __attribute__((ext_vector_type(2)));
+    f2 x;
+    __builtin_memcpy(&x, &v, 8);
+    __builtin_nontemporal_store(x, (f2 *)p);
+  } else {
+#pragma unroll
+    for (int e = 0; e < W; ++e) __builtin_nontemporal_store(v[e], p + e);
+  }
+}
+
 template <typename T, int W>
 __device__ __forceinline__ Vec<T, W> ldv_stream(const T *__restrict__ p) {
   Vec<T, W> r;
@@ -300,7 +320,7 @@ __device__ __forceinline__ void update_h_body(
     const V ex_i = ldv<T, W>(Ex + c);
     const V ez_j = ldv<T, W>(Ez + c + SJ), ex_j = ldv<T, W>(Ex + c + SJ);
     const T ey_w = Ey[c + W], ex_w = Ex[c + W];           // first cell past the vector
-    V hx = ldv<T, W>(Hx + c), hy = ldv<T, W>(Hy + c), hz = ldv<T, W>(Hz + c);
+    V hx = ldv_stream<T, W>(Hx + c), hy = ldv_stream<T, W>(Hy + c), hz = ldv_stream<T, W>(Hz + c);
     V c0, c1, c2, c3, c4, c5, c6, c7, c8;
     coeffs<T, CM, W>(g.ct, tc, 0, 0, c, g.C[CHXH], g.C[CHXEY], g.C[CHXEZ], c0, c1, c2);
     coeffs<T, CM, W>(g.ct, tc, 0, 1, c, g.C[CHYH], g.C[CHYEZ], g.C[CHYEX], c3, c4, c5);
@@ -350,8 +370,8 @@ __device__ __forceinline__ void update_h_body(
               [&] { return dx_hz[e]; }, [&] { return dy_hz[e]; });
       }
     }
-    stv<T, W>(Hx + c, hx);
-    if (inner) { stv<T, W>(Hy + c, hy); stv<T, W>(Hz + c, hz); }
+    stv_stream<T, W>(Hx + c, hx);
+    if (inner) { stv_stream<T, W>(Hy + c, hy); stv_stream<T, W>(Hz + c, hz); }
     ey_i = ey_n;
     ez_i = ez_n;
   }
@@ -397,7 +417,7 @@ __device__ __forceinline__ void update_e_body(
     const V hz_i = ldv<T, W>(Hz + c), hy_i = ldv<T, W>(Hy + c), hx_i = ldv<T, W>(Hx + c);
     const V hz_j = ldv<T, W>(Hz + c - SJ), hx_j = ldv<T, W>(Hx + c - SJ);
     const T hy_w = Hy[c - 1], hx_w = Hx[c - 1];            // last cell before the vector
-    V ex = ldv<T, W>(Ex + c), ey = ldv<T, W>(Ey + c), ez = ldv<T, W>(Ez + c);
+    V ex = ldv_stream<T, W>(Ex + c), ey = ldv_stream<T, W>(Ey + c), ez = ldv_stream<T, W>(Ez + c);
     V c0, c1, c2, c3, c4, c5, c6, c7, c8;
     coeffs<T, CM, W>(g.ct, tc, 3, 0, c, g.C[CEXE], g.C[CEXHZ], g.C[CEXHY], c0, c1, c2);
     coeffs<T, CM, W>(g.ct, tc, 3, 1, c, g.C[CEYE], g.C[CEYHX], g.C[CEYHZ], c3, c4, c5);
@@ -455,9 +475,9 @@ __device__ __forceinline__ void update_e_body(
               [&] { return dx_ez[e]; }, [&] { return dy_ez[e]; });
       }
     }
-    if (i < nx) stv<T, W>(Ex + c, ex);
-    stv<T, W>(Ey + c, ey);
-    stv<T, W>(Ez + c, ez);
+    if (i < nx) stv_stream<T, W>(Ex + c, ex);
+    stv_stream<T, W>(Ey + c, ey);
+    stv_stream<T, W>(Ez + c, ez);
 
     if (ghost && i == 0) {
       const int cg = nx * SI + p;
