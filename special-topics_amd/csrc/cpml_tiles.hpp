@@ -1,0 +1,125 @@
+// Work decomposition of the separate CPML pass: which cells it owns for each CPML
+// placement and how they are cut into workgroup tiles.  Plain C++17 with no HIP
+// dependency, so a host-only program can check it; the per-lane decode is constexpr,
+// which device code may call as well.
+//
+// The pass covers every cell a CPML face of the phase acts on that the bulk kernels
+// leave to it.  Each axis is cut at n, n + 1 and every face's slab bounds, so every
+// product box of the pieces lies wholly inside or outside each face slab: its face
+// set is one mask.  Boxes are cut into tiles of one 256-lane workgroup: 1 << ks lanes
+// along the contiguous axis k, 1 << js rows along j and the remaining
+// 256 >> (ks + js) planes along i.  A tile record holds
+//   x = i0 | mask << 16 | ks << 22 | js << 26   (mask bit 2 * axis + side, n = 0, p = 1)
+//   y = j0 | jend << 16,  z = k0 | kend << 16,  w = iend
+// so a workgroup decodes its cells from one scalar load, and which faces and terms
+// apply is uniform across it.
+#pragma once
+#include <algorithm>
+#include <vector>
+
+namespace fdtd {
+
+// CPML placement (a tuning choice; results are identical): the separate k_cpml pass
+// for every face, the z-face slabs fused into the bulk kernels, or every face fused.
+constexpr int CPML_SEPARATE = 0, CPML_FUSE_Z = 1, CPML_FUSE_ALL = 2;
+
+struct CpmlTile {
+  unsigned x, y, z, w;
+};
+
+// The cell of `lane` in tile t; false for lanes past the tile's box.
+constexpr bool cpml_tile_cell(const CpmlTile &t, int lane, unsigned &mask, int &i, int &j, int &k) {
+  const unsigned ks = (t.x >> 22) & 15u, js = (t.x >> 26) & 15u;
+  mask = (t.x >> 16) & 63u;
+  k = (int)(t.z & 0xffffu) + (lane & ((1 << ks) - 1));
+  j = (int)(t.y & 0xffffu) + ((lane >> ks) & ((1 << js) - 1));
+  i = (int)(t.x & 0xffffu) + (lane >> (ks + js));
+  return k < (int)(t.z >> 16) && j < (int)(t.y >> 16) && i < (int)t.w;
+}
+
+// Grid size and the CPML faces (xn, xp, yn, yp, zn, zp; nc == 0 when absent) with the
+// first H and E plane of each slab.
+struct CpmlLayout {
+  int n[3];
+  int nc[6], m_start[6], e_start[6];
+};
+
+// Largest grid axis the 16-bit tile coordinates hold (planes 0..n).
+constexpr int CPML_TILE_MAX_AXIS = 65534;
+
+struct CpmlPiece {
+  int lo, hi;
+  unsigned mask;
+};
+
+// The pieces of axis a for phase (0 = H, 1 = E): [0, n + 1) cut at n and at every
+// face slab bound, each piece with the mask bits of the faces whose slab holds it.
+inline std::vector<CpmlPiece> cpml_pieces(const CpmlLayout &L, int phase, int a) {
+  const int n = L.n[a];
+  std::vector<int> cut = {0, n, n + 1};
+  for (int s = 0; s < 2; ++s) {
+    const int f = 2 * a + s;
+    if (!L.nc[f]) continue;
+    const int lo = phase == 0 ? L.m_start[f] : L.e_start[f];
+    cut.push_back(lo);
+    cut.push_back(lo + L.nc[f]);
+  }
+  std::sort(cut.begin(), cut.end());
+  cut.erase(std::unique(cut.begin(), cut.end()), cut.end());
+  std::vector<CpmlPiece> out;
+  for (size_t q = 0; q + 1 < cut.size(); ++q) {
+    CpmlPiece p{cut[q], cut[q + 1], 0u};
+    for (int s = 0; s < 2; ++s) {
+      const int f = 2 * a + s;
+      const int lo = phase == 0 ? L.m_start[f] : L.e_start[f];
+      if (L.nc[f] && lo <= p.lo && p.hi <= lo + L.nc[f]) p.mask |= 1u << f;
+    }
+    out.push_back(p);
+  }
+  return out;
+}
+
+// The tiles of the cells the CPML pass owns for one phase and placement: every cell
+// of a face slab with CPML_SEPARATE; with CPML_FUSE_Z all but the z-slab cells of the
+// planes the bulk kernels cover (H: all; E: i < nx); none with CPML_FUSE_ALL.  Lanes
+// go along k (the box's k extent rounded up to a power of two, at most 64), then rows
+// along j (its j extent rounded up, up to what is left of 256), then planes along i.
+// Tiles are listed in groups of up to eight (j, k) tiles, plane-minor: workgroups are
+// dealt round-robin to the 8 XCDs, so in a full group one (j, k) tile's consecutive
+// planes -- which read each other's plane as the x neighbour -- run on one XCD and
+// share its L2.  The grid axes must not exceed CPML_TILE_MAX_AXIS.
+inline void build_cpml_tiles(const CpmlLayout &L, int phase, int mode, std::vector<CpmlTile> &out) {
+  if (mode == CPML_FUSE_ALL) return;
+  std::vector<CpmlPiece> P[3];
+  for (int a = 0; a < 3; ++a) P[a] = cpml_pieces(L, phase, a);
+  const int i_end = phase == 0 ? L.n[0] + 1 : L.n[0];
+  auto log2_up = [](int v, int cap) {
+    int s = 0;
+    while ((1 << s) < v && s < cap) ++s;
+    return s;
+  };
+  for (const auto &px : P[0])
+    for (const auto &py : P[1])
+      for (const auto &pz : P[2]) {
+        const unsigned mask = px.mask | py.mask | pz.mask;
+        if (!mask) continue;
+        if (mode == CPML_FUSE_Z && pz.mask && px.hi <= i_end) continue;
+        const int ks = log2_up(pz.hi - pz.lo, 6);
+        const int js = log2_up(py.hi - py.lo, 8 - ks);
+        const int kt = 1 << ks, jt = 1 << js, it = 256 >> (ks + js);
+        const int nbk = (pz.hi - pz.lo + kt - 1) / kt, nbj = (py.hi - py.lo + jt - 1) / jt;
+        const int nt = nbj * nbk;
+        for (int g0 = 0; g0 < nt; g0 += 8)
+          for (int i0 = px.lo; i0 < px.hi; i0 += it)
+            for (int t = g0; t < std::min(g0 + 8, nt); ++t) {
+              const int j0 = py.lo + (t / nbk) * jt, k0 = pz.lo + (t % nbk) * kt;
+              const int j1 = std::min(j0 + jt, py.hi), k1 = std::min(k0 + kt, pz.hi);
+              const int i1 = std::min(i0 + it, px.hi);
+              out.push_back(CpmlTile{(unsigned)i0 | mask << 16 | (unsigned)ks << 22 | (unsigned)js << 26,
+                                     (unsigned)j0 | (unsigned)j1 << 16, (unsigned)k0 | (unsigned)k1 << 16,
+                                     (unsigned)i1});
+            }
+      }
+}
+
+}  // namespace fdtd

@@ -121,7 +121,14 @@ struct FdtdHipContext {
   int timing = 0;
   hipEvent_t ev[8] = {};
   FdtdHipKernelTimes times{};
-  CpmlBoxes cpml_h{}, cpml_e{};
+  // CPML pass tiles of both phases for each CPML placement, in one device buffer,
+  // rebuilt when a face is uploaded; cpml_tiles / ncpml name the current placement's.
+  CpmlTile *tile_buf = nullptr;
+  size_t tile_cap = 0;
+  int faces_version = 0, tiles_version = -1;
+  size_t tile_off[3][2] = {}, tile_n[3][2] = {};
+  const CpmlTile *cpml_tiles[2] = {};
+  long ncpml[2] = {};
 
   // Plane 0 of array c / w (one guard plane into its allocation slot).
   void *field_ptr(int c) const { return (char *)fields + c * stride_bytes + (size_t)SI * esz; }
@@ -442,65 +449,47 @@ void launch_e(FdtdHipContext *c, const Grid<T> &g, const MarchGeom &m, int ghost
   }
 }
 
-// Disjoint boxes covering every cell a CPML face of the phase (0 = H, 1 = E) acts on
-// that the bulk kernels leave to k_cpml: all of them with CPML_SEPARATE; with
-// CPML_FUSE_Z all but the z-slab cells of the planes the bulk kernels cover (H: all;
-// E: i < nx); none with CPML_FUSE_ALL.
-CpmlBoxes cpml_boxes(const FdtdHipContext *c, int phase) {
-  const int n[3] = {c->d.nx, c->d.ny, c->d.nz};
-  std::vector<std::pair<int, int>> in[3], out[3];
-  for (int a = 0; a < 3; ++a) {
-    for (int s = 0; s < 2; ++s) {
-      const auto &f = c->face[2 * a + s];
-      if (!f.nc) continue;
-      const int lo = phase == 0 ? f.m_start : f.e_start;
-      in[a].push_back({lo, lo + f.nc});
-    }
-    std::sort(in[a].begin(), in[a].end());
-    int pos = 0;
-    for (auto &iv : in[a]) {
-      if (iv.first > pos) out[a].push_back({pos, iv.first});
-      pos = std::max(pos, iv.second);
-    }
-    if (pos < n[a] + 1) out[a].push_back({pos, n[a] + 1});
+// The CPML faces of a context as the tile builder takes them.
+CpmlLayout cpml_layout(const FdtdHipContext *c) {
+  CpmlLayout L{};
+  L.n[0] = c->d.nx; L.n[1] = c->d.ny; L.n[2] = c->d.nz;
+  for (int f = 0; f < 6; ++f) {
+    L.nc[f] = c->face[f].nc;
+    L.m_start[f] = c->face[f].m_start;
+    L.e_start[f] = c->face[f].e_start;
   }
-  CpmlBoxes bx{};
-  auto add = [&](std::pair<int, int> x, std::pair<int, int> y, std::pair<int, int> z) {
-    if (x.second <= x.first || y.second <= y.first || z.second <= z.first || bx.nbox >= MAX_CPML_BOXES) return;
-    const long long end = bx.start[bx.nbox] + (long long)(x.second - x.first) * (y.second - y.first) *
-                                                 (z.second - z.first);
-    if (end > INT_MAX) { bx.overflow = 1; return; }
-    const int b = bx.nbox++;
-    bx.lo[b][0] = x.first; bx.lo[b][1] = y.first; bx.lo[b][2] = z.first;
-    bx.ext[b][0] = x.second - x.first; bx.ext[b][1] = y.second - y.first; bx.ext[b][2] = z.second - z.first;
-    bx.start[b + 1] = (int)end;
-  };
-  const std::pair<int, int> fy{0, n[1] + 1}, fz{0, n[2] + 1};
-  if (fuse_cpml(c) == CPML_FUSE_ALL) return bx;
-  if (fuse_cpml(c) == CPML_SEPARATE) {
-    for (auto &x : in[0]) add(x, fy, fz);
-    for (auto &x : out[0]) for (auto &y : in[1]) add(x, y, fz);
-    for (auto &x : out[0]) for (auto &y : out[1]) for (auto &z : in[2]) add(x, y, z);
-    return bx;
-  }
-  for (auto &x : in[0]) for (auto &z : out[2]) add(x, fy, z);
-  for (auto &x : out[0]) for (auto &y : in[1]) for (auto &z : out[2]) add(x, y, z);
-  const int i_end = phase == 0 ? n[0] + 1 : n[0];          // first plane the bulk does not cover
-  for (int a = 0; a < 2; ++a)
-    for (auto &x : a == 0 ? in[0] : out[0]) {
-      const std::pair<int, int> xr{std::max(x.first, i_end), x.second};
-      for (auto &z : in[2]) add(xr, fy, z);
-    }
-  return bx;
+  return L;
 }
 
-// Boxes of both phases.  The separate CPML pass enumerates its cells with 32-bit
-// indices; a region of 2^31 cells or more is refused rather than cut short.
-int set_cpml_boxes(FdtdHipContext *c) {
-  c->cpml_h = cpml_boxes(c, 0);
-  c->cpml_e = cpml_boxes(c, 1);
-  if (c->cpml_h.overflow || c->cpml_e.overflow)
-    return fail("CPML region of 2^31 cells or more: too large for the separate CPML pass (use fuse_cpml 2)");
+// Tiles of both phases for the current placement, building every placement's lists
+// when the faces changed.  Tile records hold coordinates in 16 bits.
+int set_cpml_tiles(FdtdHipContext *c) {
+  if (c->tiles_version != c->faces_version) {
+    if (std::max({c->d.nx, c->d.ny, c->d.nz}) > CPML_TILE_MAX_AXIS)
+      return fail("CPML pass: grid axes of more than %d cells are not supported", CPML_TILE_MAX_AXIS);
+    const CpmlLayout L = cpml_layout(c);
+    std::vector<CpmlTile> all;
+    for (int mode = 0; mode < 3; ++mode)
+      for (int ph = 0; ph < 2; ++ph) {
+        c->tile_off[mode][ph] = all.size();
+        build_cpml_tiles(L, ph, mode, all);
+        c->tile_n[mode][ph] = all.size() - c->tile_off[mode][ph];
+      }
+    if (all.size() > c->tile_cap) {
+      (void)hipFree(c->tile_buf);
+      c->tile_buf = nullptr;
+      c->tile_cap = 0;
+      HIP_TRY(hipMalloc((void **)&c->tile_buf, all.size() * sizeof(CpmlTile)));
+      c->tile_cap = all.size();
+    }
+    if (!all.empty() && upload_vec(c, c->tile_buf, all)) return FDTD_HIP_ERR;
+    c->tiles_version = c->faces_version;
+  }
+  const int mode = fuse_cpml(c);
+  for (int ph = 0; ph < 2; ++ph) {
+    c->cpml_tiles[ph] = c->tile_buf ? c->tile_buf + c->tile_off[mode][ph] : nullptr;
+    c->ncpml[ph] = (long)c->tile_n[mode][ph];
+  }
   return FDTD_HIP_OK;
 }
 
@@ -576,15 +565,13 @@ int launch_step(FdtdHipContext *c, int ts, const int *tsp = nullptr) {
   if (t) HIP_TRY(hipEventRecord(c->ev[0], c->stream));
   launch_h<T>(c, g, mh, fs);
   if (t) HIP_TRY(hipEventRecord(c->ev[1], c->stream));
-  if (c->cpml_h.nbox)
-    hipLaunchKernelGGL(k_cpml_h<T>, dim3((unsigned)(((long long)c->cpml_h.start[c->cpml_h.nbox] + 255) / 256)), block, 0,
-                       c->stream, g, fs, c->cpml_h);
+  if (c->ncpml[0])
+    hipLaunchKernelGGL(k_cpml_h<T>, dim3((unsigned)c->ncpml[0]), block, 0, c->stream, g, fs, c->cpml_tiles[0]);
   if (t) HIP_TRY(hipEventRecord(c->ev[2], c->stream));
   launch_e<T>(c, g, me, ghost, fs);
   if (t) HIP_TRY(hipEventRecord(c->ev[3], c->stream));
-  if (c->cpml_e.nbox)
-    hipLaunchKernelGGL(k_cpml_e<T>, dim3((unsigned)(((long long)c->cpml_e.start[c->cpml_e.nbox] + 255) / 256)), block, 0,
-                       c->stream, g, fs, c->cpml_e);
+  if (c->ncpml[1])
+    hipLaunchKernelGGL(k_cpml_e<T>, dim3((unsigned)c->ncpml[1]), block, 0, c->stream, g, fs, c->cpml_tiles[1]);
   if (t) HIP_TRY(hipEventRecord(c->ev[4], c->stream));
   if (c->ports.nvs + c->ports.nsv + c->ports.nsi + c->ports.npr > 0) {
     PortState ps = c->ports;
@@ -810,6 +797,7 @@ int fdtd_hip_upload_cpml(FdtdHipContext *c, int faceId, int nc, const double *b_
     }
   }
   F.nc = nc;
+  ++c->faces_version;
   return FDTD_HIP_OK;
 }
 
@@ -935,7 +923,7 @@ int fdtd_hip_run_batch_async(FdtdHipContext *c, int tsStart, int count) {
     return fail("run_batch: steps [%d,%d) outside [0,%d)", tsStart, tsStart + count, c->d.numberOfTimeSteps);
   if (c->ndft && c->ports.nsv + c->ports.nsi == 0) return fail("run_batch: port DFT without observers");
   if (ensure_traces(c) || finalize_coefficients(c)) return FDTD_HIP_ERR;
-  if (set_cpml_boxes(c)) return FDTD_HIP_ERR;
+  if (set_cpml_tiles(c)) return FDTD_HIP_ERR;
   if (ntff_blocked(c) && ensure_ntff_buffer(c)) return FDTD_HIP_ERR;
   if (c->fp32() ? run_steps<float>(c, tsStart, count) : run_steps<double>(c, tsStart, count)) return FDTD_HIP_ERR;
   // Leave the accumulators complete at the end of every batch.
@@ -1267,7 +1255,7 @@ int fdtd_hip_autotune(FdtdHipContext *c, int steps_per_candidate, FdtdHipTuning 
   if (steps_per_candidate < 1) steps_per_candidate = 2;
   const int steps = std::min(steps_per_candidate + 1, c->d.numberOfTimeSteps);
   if (ensure_traces(c) || build_coefficient_tables(c)) return FDTD_HIP_ERR;
-  if (set_cpml_boxes(c)) return FDTD_HIP_ERR;
+  if (set_cpml_tiles(c)) return FDTD_HIP_ERR;
   if (ntff_blocked(c) && ensure_ntff_buffer(c)) return FDTD_HIP_ERR;
   const FdtdHipTuning saved = c->tune;
   const int saved_timing = c->timing;
@@ -1286,7 +1274,7 @@ int fdtd_hip_autotune(FdtdHipContext *c, int steps_per_candidate, FdtdHipTuning 
   FdtdHipTuning best_t = saved;
   for (const auto &t : cands) {
     c->tune = t;
-    if (set_cpml_boxes(c)) return FDTD_HIP_ERR;
+    if (set_cpml_tiles(c)) return FDTD_HIP_ERR;
     c->coef_mode = -1;
     if (finalize_coefficients(c)) return FDTD_HIP_ERR;
     c->timing = 1;
@@ -1344,6 +1332,7 @@ void fdtd_hip_destroy(FdtdHipContext *c) {
   for (void *p : c->small_allocs) (void)hipFree(p);
   for (void *p : c->coef_tab) (void)hipFree(p);
   (void)hipFree(c->coef_idx);
+  (void)hipFree(c->tile_buf);
   (void)hipFree(c->ff_buf);
   (void)hipFree(c->ntff_buf);
   drop_graph(c);
@@ -1434,7 +1423,7 @@ int multi_prepare(FdtdHipMulti *m) {
         drop_graph(c);
       }
       if (ensure_traces(c) || finalize_coefficients(c)) return FDTD_HIP_ERR;
-      if (set_cpml_boxes(c)) return FDTD_HIP_ERR;
+      if (set_cpml_tiles(c)) return FDTD_HIP_ERR;
       if (ntff_blocked(c) && ensure_ntff_buffer(c)) return FDTD_HIP_ERR;
     }
     bool same = true;
@@ -1466,8 +1455,8 @@ int multi_prepare(FdtdHipMulti *m) {
     d.ghost = c->d.boundary == FDTD_HIP_BOUNDARY_REFERENCE;
     d.mh = march_geom(c, c->d.nx + 1);
     d.me = march_geom(c, d.ghost ? c->d.nx : c->d.nx + 1);
-    d.bh = c->cpml_h;
-    d.be = c->cpml_e;
+    d.th = c->cpml_tiles[0];
+    d.te = c->cpml_tiles[1];
     d.ps = c->ports;
     d.ps.ndft = c->ndft; d.ps.w = c->dft_w; d.ps.dft = c->dft_acc;
     d.has_ports = c->ports.nvs + c->ports.nsv + c->ports.nsi + c->ports.npr > 0;
@@ -1479,8 +1468,7 @@ int multi_prepare(FdtdHipMulti *m) {
       const FarfieldState &f = c->ff;
       tf = (2 * ((long)f.nj * f.nk + (long)f.ni * f.nk + (long)f.ni * f.nj) + 255) / 256;
     }
-    const long tch = (long)(((long long)c->cpml_h.start[c->cpml_h.nbox] + 255) / 256);
-    const long tce = (long)(((long long)c->cpml_e.start[c->cpml_e.nbox] + 255) / 256);
+    const long tch = c->ncpml[0], tce = c->ncpml[1];
     if (th > (long)MULTI_LOCAL_MASK || te > (long)MULTI_LOCAL_MASK || tf > (long)MULTI_LOCAL_MASK ||
         tch > (long)MULTI_LOCAL_MASK || tce > (long)MULTI_LOCAL_MASK)
       return fail("multi: variant %d is too large for a batched launch", v);

@@ -19,6 +19,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "cpml_tiles.hpp"
+
 namespace fdtd {
 
 enum CoeffId {
@@ -226,26 +228,12 @@ __device__ __forceinline__ T cpml_terms(const FaceSet<T> &fs, T v, int i, int j,
   return v;
 }
 
-// The same on the stored element F[c]; loads and stores it only if a face applies.
-template <typename T, int A1, int S1, int A2, int S2, bool MAG, typename D1, typename D2>
-__device__ __forceinline__ void cpml_component(const FaceSet<T> &fs, T *__restrict__ F, int c,
-                                               int i, int j, int k, int d1, int d2, D1 diff1, D2 diff2) {
-  const int a0 = cpml_slot<T, A1, MAG>(fs, 0, i, j, k), a1 = cpml_slot<T, A1, MAG>(fs, 1, i, j, k);
-  const int b0 = cpml_slot<T, A2, MAG>(fs, 0, i, j, k), b1 = cpml_slot<T, A2, MAG>(fs, 1, i, j, k);
-  if (a0 < 0 && a1 < 0 && b0 < 0 && b1 < 0) return;
-  F[c] = cpml_terms<T, A1, S1, A2, S2, MAG>(fs, F[c], i, j, k, d1, d2, diff1, diff2);
-}
-
 // Whether coordinate v along AXIS lies in a slab of that axis's faces (MAG: H planes,
 // else E planes).
 template <typename T, int AXIS, bool MAG>
 __device__ __forceinline__ bool in_slab(const FaceSet<T> &fs, int v) {
   return cpml_slot<T, AXIS, MAG>(fs, 0, v, v, v) >= 0 || cpml_slot<T, AXIS, MAG>(fs, 1, v, v, v) >= 0;
 }
-
-// CPML placement (a tuning choice; results are identical): the separate k_cpml pass
-// for every face, the z-face slabs fused into the bulk kernels, or every face fused.
-constexpr int CPML_SEPARATE = 0, CPML_FUSE_Z = 1, CPML_FUSE_ALL = 2;
 
 // Whether the bulk kernel applies the CPML terms of (i,j,k) itself.
 template <typename T, int FC, bool MAG>
@@ -531,90 +519,144 @@ __global__ void __launch_bounds__(256) k_update_e(Grid<T> g, MarchGeom m, int gh
 
 // ------------------------------------------------------------------ CPML pass --
 
-// The cells any CPML face of a phase touches, as disjoint boxes: the x-slabs over the
-// full y,z extent, then the y-slabs outside the x-slabs, then the z-slabs outside
-// both.  Boxes are enumerated k-fastest through a flat index, one cell per thread,
-// one component after another.  Two restructurings were measured slower, both through
-// the occupancy their registers cost: four cells per thread with each component's
-// loads issued before its stores, and all three components' loads issued before any
-// store.
-constexpr int MAX_CPML_BOXES = 32;
-struct CpmlBoxes {
-  int nbox;
-  int start[MAX_CPML_BOXES + 1];   // flat start of each box; start[nbox] = total cells (< 2^31)
-  int lo[MAX_CPML_BOXES][3], ext[MAX_CPML_BOXES][3];
-  int overflow;                    // set by the host builder when the cells exceed 2^31 - 1
+// The pass's tiles (CpmlTile, cpml_tile_cell) are described in cpml_tiles.hpp.
+
+// One component's CPML work at one cell, staged: every load first (field, the
+// operands of the curl differences, Psi, CPsi, profile), then the differences, the
+// Psi recurrences and field corrections in the reference's order (faces of axis A1 n
+// then p, then of A2; cpml.js:470,484) and the stores.  Issuing the loads of all
+// three components before any arithmetic or store keeps them in flight together.
+// The curl difference along an axis is P[c + hi] - P[c + lo].
+template <typename T>
+struct CpmlDiff {
+  const T *P;
+  int hi, lo;
 };
 
-// Box coordinates of flat index q, in 32-bit unsigned arithmetic (a 64-bit division
-// costs several times as many instructions).
-__device__ __forceinline__ bool cpml_cell(const CpmlBoxes &bx, long long q, int &i, int &j, int &k) {
-  if (q >= bx.start[bx.nbox]) return false;
-  const unsigned u = (unsigned)q;
-  int b = 0;
-  while (u >= (unsigned)bx.start[b + 1]) ++b;
-  const unsigned r = u - (unsigned)bx.start[b];
-  const unsigned ek = (unsigned)bx.ext[b][2], ej = (unsigned)bx.ext[b][1];
-  const unsigned rk = r / ek, rj = rk / ej;
-  k = bx.lo[b][2] + (int)(r - rk * ek);
-  j = bx.lo[b][1] + (int)(rk - rj * ej);
-  i = bx.lo[b][0] + (int)rj;
-  return true;
+template <typename T>
+struct CpmlStage {
+  T v, u1, w1, u2, w2;
+  T ps[4], cp[4], b[4], a[4];
+};
+
+template <typename T, int A, int SLOT, bool MAG>
+__device__ __forceinline__ void cpml_load_face(const FaceSet<T> &fs, int s, int i, int j, int k, int d1, int d2,
+                                               T &ps, T &cp, T &b, T &a) {
+  const Face<T> &F = fs.f[2 * A + s];
+  const int ci = (A == 0 ? i : (A == 1 ? j : k)) - (MAG ? F.m_start : F.e_start);
+  const int q = psi_offset<A>(i, j, k, ci, F.nc, d1, d2);
+  ps = F.psi[SLOT][q];
+  cp = F.cpsi[SLOT][q];
+  b = MAG ? F.b_m[ci] : F.b_e[ci];
+  a = MAG ? F.a_m[ci] : F.a_e[ci];
 }
 
-// Magnetic CPML terms of every H element in the CPML slabs, faces in the reference's
-// order per component (x, then y, then z; cpml.js:456-724).  Runs after the H bulk
-// update, as the reference's updateMagneticCPML does; the E differences are this
-// step's E, which no kernel changes in between.
-template <typename T>
-__device__ __forceinline__ void cpml_h_body(const Grid<T> &g, const FaceSet<T> &fs, const CpmlBoxes &bx, long long q) {
+template <typename T, int A, int SLOT, bool MAG>
+__device__ __forceinline__ T cpml_store_face(const FaceSet<T> &fs, int s, int i, int j, int k, int d1, int d2,
+                                             T v, T diff, T ps, T cp, T b, T a) {
+  const Face<T> &F = fs.f[2 * A + s];
+  const int ci = (A == 0 ? i : (A == 1 ? j : k)) - (MAG ? F.m_start : F.e_start);
+  const T p = b * ps + a * diff;
+  F.psi[SLOT][psi_offset<A>(i, j, k, ci, F.nc, d1, d2)] = p;
+  return v + cp * p;
+}
+
+template <typename T, int A1, int S1, int A2, int S2, bool MAG>
+__device__ __forceinline__ void cpml_stage_load(const FaceSet<T> &fs, unsigned mask, const T *F, int c, int i, int j,
+                                                int k, int d1, int d2, CpmlDiff<T> x1, CpmlDiff<T> x2,
+                                                CpmlStage<T> &L) {
+  L.v = F[c];
+  if (mask & (3u << (2 * A1))) { L.u1 = x1.P[c + x1.hi]; L.w1 = x1.P[c + x1.lo]; }
+  if (mask & (3u << (2 * A2))) { L.u2 = x2.P[c + x2.hi]; L.w2 = x2.P[c + x2.lo]; }
+  if (mask & (1u << (2 * A1))) cpml_load_face<T, A1, S1, MAG>(fs, 0, i, j, k, d1, d2, L.ps[0], L.cp[0], L.b[0], L.a[0]);
+  if (mask & (2u << (2 * A1))) cpml_load_face<T, A1, S1, MAG>(fs, 1, i, j, k, d1, d2, L.ps[1], L.cp[1], L.b[1], L.a[1]);
+  if (mask & (1u << (2 * A2))) cpml_load_face<T, A2, S2, MAG>(fs, 0, i, j, k, d1, d2, L.ps[2], L.cp[2], L.b[2], L.a[2]);
+  if (mask & (2u << (2 * A2))) cpml_load_face<T, A2, S2, MAG>(fs, 1, i, j, k, d1, d2, L.ps[3], L.cp[3], L.b[3], L.a[3]);
+}
+
+template <typename T, int A1, int S1, int A2, int S2, bool MAG>
+__device__ __forceinline__ void cpml_stage_store(const FaceSet<T> &fs, unsigned mask, T *F, int c, int i, int j,
+                                                 int k, int d1, int d2, const CpmlStage<T> &L) {
 #pragma clang fp contract(off)
+  T v = L.v;
+  const T e1 = L.u1 - L.w1, e2 = L.u2 - L.w2;
+  if (mask & (1u << (2 * A1)))
+    v = cpml_store_face<T, A1, S1, MAG>(fs, 0, i, j, k, d1, d2, v, e1, L.ps[0], L.cp[0], L.b[0], L.a[0]);
+  if (mask & (2u << (2 * A1)))
+    v = cpml_store_face<T, A1, S1, MAG>(fs, 1, i, j, k, d1, d2, v, e1, L.ps[1], L.cp[1], L.b[1], L.a[1]);
+  if (mask & (1u << (2 * A2)))
+    v = cpml_store_face<T, A2, S2, MAG>(fs, 0, i, j, k, d1, d2, v, e2, L.ps[2], L.cp[2], L.b[2], L.a[2]);
+  if (mask & (2u << (2 * A2)))
+    v = cpml_store_face<T, A2, S2, MAG>(fs, 1, i, j, k, d1, d2, v, e2, L.ps[3], L.cp[3], L.b[3], L.a[3]);
+  F[c] = v;
+}
+
+// Face bits of each axis in a tile mask.
+constexpr unsigned CPML_X = 3u, CPML_Y = 3u << 2, CPML_Z = 3u << 4;
+
+// Magnetic CPML terms of one tile's H elements, faces in the reference's order per
+// component (x, then y, then z; cpml.js:456-724).  Runs after the H bulk update, as
+// the reference's updateMagneticCPML does; the E differences are this step's E,
+// which no kernel changes in between.
+template <typename T>
+__device__ __forceinline__ void cpml_h_tile(const Grid<T> &g, const FaceSet<T> &fs, const CpmlTile &t, int lane) {
+#pragma clang fp contract(off)
+  unsigned mask;
   int i, j, k;
-  if (!cpml_cell(bx, q, i, j, k)) return;
+  if (!cpml_tile_cell(t, lane, mask, i, j, k)) return;
   const int nx = g.nx, ny = g.ny, nz = g.nz, SJ = g.SJ, SI = g.SI;
   const int c = i * SI + j * SJ + k;
   const T *Ex = g.Ex, *Ey = g.Ey, *Ez = g.Ez;
-  if (j < ny && k < nz)                                    // Hx: y then z faces
-    cpml_component<T, 1, 0, 2, 0, true>(fs, g.Hx, c, i, j, k, ny, nz,
-        [&] { return Ez[c + SJ] - Ez[c]; }, [&] { return Ey[c + 1] - Ey[c]; });
-  if (i < nx && k < nz)                                    // Hy: x then z faces
-    cpml_component<T, 0, 0, 2, 1, true>(fs, g.Hy, c, i, j, k, ny + 1, nz,
-        [&] { return Ez[c + SI] - Ez[c]; }, [&] { return Ex[c + 1] - Ex[c]; });
-  if (i < nx && j < ny)                                    // Hz: x then y faces
-    cpml_component<T, 0, 1, 1, 1, true>(fs, g.Hz, c, i, j, k, ny, nz + 1,
-        [&] { return Ey[c + SI] - Ey[c]; }, [&] { return Ex[c + SJ] - Ex[c]; });
+  const bool hx = (mask & (CPML_Y | CPML_Z)) && j < ny && k < nz;   // Hx: y then z faces
+  const bool hy = (mask & (CPML_X | CPML_Z)) && i < nx && k < nz;   // Hy: x then z faces
+  const bool hz = (mask & (CPML_X | CPML_Y)) && i < nx && j < ny;   // Hz: x then y faces
+  CpmlStage<T> Lx, Ly, Lz;
+  if (hx)
+    cpml_stage_load<T, 1, 0, 2, 0, true>(fs, mask, g.Hx, c, i, j, k, ny, nz, {Ez, SJ, 0}, {Ey, 1, 0}, Lx);
+  if (hy)
+    cpml_stage_load<T, 0, 0, 2, 1, true>(fs, mask, g.Hy, c, i, j, k, ny + 1, nz, {Ez, SI, 0}, {Ex, 1, 0}, Ly);
+  if (hz)
+    cpml_stage_load<T, 0, 1, 1, 1, true>(fs, mask, g.Hz, c, i, j, k, ny, nz + 1, {Ey, SI, 0}, {Ex, SJ, 0}, Lz);
+  if (hx) cpml_stage_store<T, 1, 0, 2, 0, true>(fs, mask, g.Hx, c, i, j, k, ny, nz, Lx);
+  if (hy) cpml_stage_store<T, 0, 0, 2, 1, true>(fs, mask, g.Hy, c, i, j, k, ny + 1, nz, Ly);
+  if (hz) cpml_stage_store<T, 0, 1, 1, 1, true>(fs, mask, g.Hz, c, i, j, k, ny, nz + 1, Lz);
 }
 
 // Electric CPML terms (cpml.js:727-1003), after the E bulk update.  Covers the whole
 // E extents, so the i = nx wall of Ey/Ez gets its z / y terms here whichever
 // boundary mode updated (or skipped) its bulk value.
 template <typename T>
-__device__ __forceinline__ void cpml_e_body(const Grid<T> &g, const FaceSet<T> &fs, const CpmlBoxes &bx, long long q) {
+__device__ __forceinline__ void cpml_e_tile(const Grid<T> &g, const FaceSet<T> &fs, const CpmlTile &t, int lane) {
 #pragma clang fp contract(off)
+  unsigned mask;
   int i, j, k;
-  if (!cpml_cell(bx, q, i, j, k)) return;
+  if (!cpml_tile_cell(t, lane, mask, i, j, k)) return;
   const int nx = g.nx, ny = g.ny, nz = g.nz, SJ = g.SJ, SI = g.SI;
   const int c = i * SI + j * SJ + k;
   const T *Hx = g.Hx, *Hy = g.Hy, *Hz = g.Hz;
-  if (i < nx)                                              // Ex: y then z faces
-    cpml_component<T, 1, 3, 2, 2, false>(fs, g.Ex, c, i, j, k, ny + 1, nz + 1,
-        [&] { return Hz[c] - Hz[c - SJ]; }, [&] { return Hy[c] - Hy[c - 1]; });
-  if (j < ny)                                              // Ey: x then z faces
-    cpml_component<T, 0, 2, 2, 3, false>(fs, g.Ey, c, i, j, k, ny, nz + 1,
-        [&] { return Hz[c] - Hz[c - SI]; }, [&] { return Hx[c] - Hx[c - 1]; });
-  if (k < nz)                                              // Ez: x then y faces
-    cpml_component<T, 0, 3, 1, 2, false>(fs, g.Ez, c, i, j, k, ny + 1, nz,
-        [&] { return Hy[c] - Hy[c - SI]; }, [&] { return Hx[c] - Hx[c - SJ]; });
+  const bool ex = (mask & (CPML_Y | CPML_Z)) && i < nx;              // Ex: y then z faces
+  const bool ey = (mask & (CPML_X | CPML_Z)) && j < ny;              // Ey: x then z faces
+  const bool ez = (mask & (CPML_X | CPML_Y)) && k < nz;              // Ez: x then y faces
+  CpmlStage<T> Lx, Ly, Lz;
+  if (ex)
+    cpml_stage_load<T, 1, 3, 2, 2, false>(fs, mask, g.Ex, c, i, j, k, ny + 1, nz + 1, {Hz, 0, -SJ}, {Hy, 0, -1}, Lx);
+  if (ey)
+    cpml_stage_load<T, 0, 2, 2, 3, false>(fs, mask, g.Ey, c, i, j, k, ny, nz + 1, {Hz, 0, -SI}, {Hx, 0, -1}, Ly);
+  if (ez)
+    cpml_stage_load<T, 0, 3, 1, 2, false>(fs, mask, g.Ez, c, i, j, k, ny + 1, nz, {Hy, 0, -SI}, {Hx, 0, -SJ}, Lz);
+  if (ex) cpml_stage_store<T, 1, 3, 2, 2, false>(fs, mask, g.Ex, c, i, j, k, ny + 1, nz + 1, Lx);
+  if (ey) cpml_stage_store<T, 0, 2, 2, 3, false>(fs, mask, g.Ey, c, i, j, k, ny, nz + 1, Ly);
+  if (ez) cpml_stage_store<T, 0, 3, 1, 2, false>(fs, mask, g.Ez, c, i, j, k, ny + 1, nz, Lz);
 }
 
 template <typename T>
-__global__ void __launch_bounds__(256) k_cpml_h(Grid<T> g, FaceSet<T> fs, CpmlBoxes bx) {
-  cpml_h_body<T>(g, fs, bx, (long long)blockIdx.x * blockDim.x + threadIdx.x);
+__global__ void __launch_bounds__(256) k_cpml_h(Grid<T> g, FaceSet<T> fs, const CpmlTile *__restrict__ tiles) {
+  cpml_h_tile<T>(g, fs, tiles[blockIdx.x], threadIdx.x);
 }
 
 template <typename T>
-__global__ void __launch_bounds__(256) k_cpml_e(Grid<T> g, FaceSet<T> fs, CpmlBoxes bx) {
-  cpml_e_body<T>(g, fs, bx, (long long)blockIdx.x * blockDim.x + threadIdx.x);
+__global__ void __launch_bounds__(256) k_cpml_e(Grid<T> g, FaceSet<T> fs, const CpmlTile *__restrict__ tiles) {
+  cpml_e_tile<T>(g, fs, tiles[blockIdx.x], threadIdx.x);
 }
 
 // ---------------------------------------------------------------- port epilogue --

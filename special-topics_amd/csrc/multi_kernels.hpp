@@ -21,7 +21,7 @@ struct VariantDesc {
   Grid<T> g;
   FaceSet<T> fs;
   MarchGeom mh, me;
-  CpmlBoxes bh, be;     // cells left to the CPML pass (none when every face is fused)
+  const CpmlTile *th, *te;   // CPML pass tiles of each phase (none when every face is fused)
   int ghost;
   int has_ports;
   PortState ps;
@@ -112,14 +112,14 @@ __global__ void __launch_bounds__(256) k_multi_update_e(const VariantDesc<T> *__
   update_e_body<T, CM, W, FC>(g.Ex, g.Ey, g.Ez, g.Hx, g.Hy, g.Hz, g, d.me, tc, col, ch, d.ghost, fs);
 }
 
-// The CPML pass: each variant's box cells padded to whole blocks, as for the NTFF.
+// The CPML pass: one workgroup per tile of the variant's tile list.
 template <typename T>
 __global__ void __launch_bounds__(256) k_multi_cpml_h(const VariantDesc<T> *__restrict__ vd,
                                                       const unsigned *__restrict__ map) {
   const unsigned entry = map[blockIdx.x];
   const VariantDesc<T> &d = vd[entry >> MULTI_LOCAL_BITS];
-  const long long q = (long long)(entry & MULTI_LOCAL_MASK) * blockDim.x + threadIdx.x;
-  cpml_h_body<T>(global_grid(d.g), global_faces(d.fs), d.bh, q);
+  const CpmlTile t = as_global(d.th)[entry & MULTI_LOCAL_MASK];
+  cpml_h_tile<T>(global_grid(d.g), global_faces(d.fs), t, threadIdx.x);
 }
 
 template <typename T>
@@ -127,8 +127,8 @@ __global__ void __launch_bounds__(256) k_multi_cpml_e(const VariantDesc<T> *__re
                                                       const unsigned *__restrict__ map) {
   const unsigned entry = map[blockIdx.x];
   const VariantDesc<T> &d = vd[entry >> MULTI_LOCAL_BITS];
-  const long long q = (long long)(entry & MULTI_LOCAL_MASK) * blockDim.x + threadIdx.x;
-  cpml_e_body<T>(global_grid(d.g), global_faces(d.fs), d.be, q);
+  const CpmlTile t = as_global(d.te)[entry & MULTI_LOCAL_MASK];
+  cpml_e_tile<T>(global_grid(d.g), global_faces(d.fs), t, threadIdx.x);
 }
 
 // One block per variant; variants without sources or observers return at once.

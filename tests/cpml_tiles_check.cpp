@@ -1,0 +1,115 @@
+// Host check of the CPML pass's work decomposition (csrc/cpml_tiles.hpp), built with
+// g++ by test_cpml_tiles.py.  For each layout, phase and CPML placement it decodes
+// every lane of every tile exactly as the kernels do and checks, against a
+// brute-force enumeration of the grid, that each cell the pass owns is visited once,
+// with the mask of exactly the faces whose slab holds it, and nothing else.
+//
+//   cpml_tiles_check            prints one line per case, then OK (exit 0) or FAIL
+#include "../special-topics_amd/csrc/cpml_tiles.hpp"
+
+#include <cstdio>
+#include <random>
+#include <vector>
+
+using namespace fdtd;
+
+static unsigned faces_of(const CpmlLayout &L, int phase, const int co[3]) {
+  unsigned m = 0;
+  for (int f = 0; f < 6; ++f) {
+    const int lo = phase == 0 ? L.m_start[f] : L.e_start[f];
+    if (L.nc[f] && co[f / 2] >= lo && co[f / 2] < lo + L.nc[f]) m |= 1u << f;
+  }
+  return m;
+}
+
+static int check(const CpmlLayout &L, const char *name) {
+  int bad = 0;
+  const int nx = L.n[0], ny = L.n[1], nz = L.n[2];
+  const long plane = (long)(ny + 1) * (nz + 1);
+  for (int mode = 0; mode < 3; ++mode)
+    for (int ph = 0; ph < 2; ++ph) {
+      std::vector<CpmlTile> tiles;
+      build_cpml_tiles(L, ph, mode, tiles);
+      std::vector<unsigned char> seen((size_t)(nx + 1) * plane, 0);
+      long lanes = 0, cells = 0;
+      for (const CpmlTile &t : tiles) {
+        for (int lane = 0; lane < 256; ++lane) {
+          unsigned mask = 0;
+          int i = 0, j = 0, k = 0;
+          if (!cpml_tile_cell(t, lane, mask, i, j, k)) continue;
+          ++lanes;
+          if (i < 0 || i > nx || j < 0 || j > ny || k < 0 || k > nz) {
+            if (bad++ < 5) std::printf("  %s mode %d phase %d: lane outside the grid (%d,%d,%d)\n", name, mode, ph, i, j, k);
+            continue;
+          }
+          const int co[3] = {i, j, k};
+          if (mask != faces_of(L, ph, co) && bad++ < 5)
+            std::printf("  %s mode %d phase %d: mask %x at (%d,%d,%d), expected %x\n", name, mode, ph, mask, i, j, k,
+                        faces_of(L, ph, co));
+          if (seen[i * plane + (long)j * (nz + 1) + k]++ && bad++ < 5)
+            std::printf("  %s mode %d phase %d: (%d,%d,%d) visited twice\n", name, mode, ph, i, j, k);
+        }
+      }
+      const int i_end = ph == 0 ? nx + 1 : nx;
+      for (int i = 0; i <= nx; ++i)
+        for (int j = 0; j <= ny; ++j)
+          for (int k = 0; k <= nz; ++k) {
+            const int co[3] = {i, j, k};
+            const unsigned m = faces_of(L, ph, co);
+            bool owned = m != 0;
+            if (mode == CPML_FUSE_ALL) owned = false;
+            if (mode == CPML_FUSE_Z && (m & (3u << 4)) && i < i_end) owned = false;
+            cells += owned;
+            if (owned != (seen[i * plane + (long)j * (nz + 1) + k] != 0) && bad++ < 5)
+              std::printf("  %s mode %d phase %d: (%d,%d,%d) owned=%d but visited=%d\n", name, mode, ph, i, j, k, owned,
+                          seen[i * plane + (long)j * (nz + 1) + k]);
+          }
+      std::printf("%s mode %d phase %d: %zu tiles, %ld cells, lane use %.3f\n", name, mode, ph, tiles.size(), cells,
+                  tiles.empty() ? 1.0 : (double)lanes / (256.0 * tiles.size()));
+    }
+  return bad;
+}
+
+// Slabs as the reference's builder places them: H planes from 0 and up to n, E planes
+// from 1 and up to n (npml cells each), optionally with some faces absent.
+static CpmlLayout standard(int nx, int ny, int nz, int npml, unsigned absent) {
+  CpmlLayout L{};
+  L.n[0] = nx; L.n[1] = ny; L.n[2] = nz;
+  for (int f = 0; f < 6; ++f) {
+    if (absent & (1u << f)) continue;
+    const int n = L.n[f / 2];
+    L.nc[f] = npml;
+    L.m_start[f] = f % 2 == 0 ? 0 : n - npml;
+    L.e_start[f] = f % 2 == 0 ? 1 : n - npml;
+  }
+  return L;
+}
+
+int main() {
+  int bad = 0;
+  bad += check(standard(64, 72, 80, 8, 0), "64x72x80 npml 8");
+  bad += check(standard(39, 136, 136, 8, 0), "39x136x136 npml 8");
+  bad += check(standard(21, 17, 70, 4, 1u << 3), "21x17x70 npml 4 no yp");
+  bad += check(standard(9, 11, 13, 2, (1u << 0) | (1u << 5)), "9x11x13 npml 2 no xn, zp");
+  bad += check(standard(12, 12, 12, 0, 63u), "12^3 no CPML");
+  // Seeded random layouts within the bounds upload_cpml accepts (2 nc < n, H planes in
+  // [0, n), E planes in [1, n)); slabs may sit anywhere, even overlap.
+  std::mt19937 rng(7);
+  for (int r = 0; r < 40; ++r) {
+    CpmlLayout L{};
+    for (int a = 0; a < 3; ++a) L.n[a] = 5 + (int)(rng() % 40);
+    for (int f = 0; f < 6; ++f) {
+      if (rng() % 4 == 0) continue;
+      const int n = L.n[f / 2];
+      const int nc = 1 + (int)(rng() % ((n - 1) / 2));
+      L.nc[f] = nc;
+      L.m_start[f] = (int)(rng() % (n - nc + 1));
+      L.e_start[f] = 1 + (int)(rng() % (n - nc));
+    }
+    char name[64];
+    std::snprintf(name, sizeof name, "random %d (%dx%dx%d)", r, L.n[0], L.n[1], L.n[2]);
+    bad += check(L, name);
+  }
+  std::printf(bad ? "FAIL (%d)\n" : "OK\n", bad);
+  return bad != 0;
+}
