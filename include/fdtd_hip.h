@@ -150,6 +150,13 @@ int fdtd_hip_run_batch_async(FdtdHipContext *ctx, int tsStart, int count);
 int fdtd_hip_read_traces(FdtdHipContext *ctx, int tsStart, int count, double *svOut, double *siOut);
 int fdtd_hip_query(FdtdHipContext *ctx, int *idle);
 
+/* Whether the last time step ran its E-phase tail -- the E CPML pass, the port
+ * epilogue (source injection, captures, running port DFT) and the NTFF step -- as one
+ * launch (1) or as separate launches (0).  One launch is used when the three touch
+ * disjoint field elements and per-kernel timing is off; results are identical either
+ * way.  No counterpart in the reference ABI (diagnostic). */
+int fdtd_hip_get_step_info(FdtdHipContext *ctx, int *e_tail_merged);
+
 /* Copy a field (which: 0=Hx,1=Hy,2=Hz,3=Ex,4=Ey,5=Ez) to host in the reference
  * layout; n must equal the component's length.  Replaces fdtd_read_field
  * (fdtd_cuda.h:60), FP64 output. */

@@ -1,7 +1,8 @@
 // Work decomposition of the separate CPML pass: which cells it owns for each CPML
-// placement and how they are cut into workgroup tiles.  Plain C++17 with no HIP
-// dependency, so a host-only program can check it; the per-lane decode is constexpr,
-// which device code may call as well.
+// placement and how they are cut into workgroup tiles, and which E elements it may
+// change (so the step's other E-phase work can share its launch).  Plain C++17 with
+// no HIP dependency, so a host-only program can check it; the per-lane decode is
+// constexpr, which device code may call as well.
 //
 // The pass covers every cell a CPML face of the phase acts on that the bulk kernels
 // leave to it.  Each axis is cut at n, n + 1 and every face's slab bounds, so every
@@ -15,6 +16,7 @@
 // apply is uniform across it.
 #pragma once
 #include <algorithm>
+#include <array>
 #include <vector>
 
 namespace fdtd {
@@ -120,6 +122,52 @@ inline void build_cpml_tiles(const CpmlLayout &L, int phase, int mode, std::vect
                                      (unsigned)i1});
             }
       }
+}
+
+// Whether the E CPML pass may change E component `comp` (0..2 = Ex..Ez) at (i, j, k):
+// the coordinate along one of the component's two transverse axes lies in an E slab of
+// that axis.  Conservative at the grid edges, where the component may not exist.
+inline bool cpml_e_touches(const CpmlLayout &L, int comp, int i, int j, int k) {
+  const int co[3] = {i, j, k};
+  for (int a = 0; a < 3; ++a) {
+    if (a == comp) continue;
+    for (int s = 0; s < 2; ++s) {
+      const int f = 2 * a + s;
+      if (L.nc[f] && co[a] >= L.e_start[f] && co[a] < L.e_start[f] + L.nc[f]) return true;
+    }
+  }
+  return false;
+}
+
+// An E element (component 0..2 = Ex..Ez, i, j, k).
+typedef std::array<int, 4> ECell;
+
+// Whether the E CPML pass, the port epilogue and the NTFF step of a time step touch
+// disjoint E elements, so that they may run concurrently in one launch: no element the
+// epilogue writes (sources) or reads (observers, probes) may change in the E CPML
+// pass; with an NTFF box [lo, hi] (0-based planes, both ends included) the box may not
+// reach into an E slab (its reads lie on the box surface), and no element the
+// epilogue writes may lie on that surface.
+inline bool e_tail_disjoint(const CpmlLayout &L, const std::vector<ECell> &writes, const std::vector<ECell> &reads,
+                            bool ntff, const int lo[3], const int hi[3]) {
+  for (const auto *v : {&writes, &reads})
+    for (const ECell &e : *v)
+      if (cpml_e_touches(L, e[0], e[1], e[2], e[3])) return false;
+  if (!ntff) return true;
+  for (int f = 0; f < 6; ++f) {
+    const int a = f / 2;
+    if (L.nc[f] && L.e_start[f] <= hi[a] && lo[a] < L.e_start[f] + L.nc[f]) return false;
+  }
+  for (const ECell &e : writes) {
+    bool inside = true, on_face = false;
+    for (int a = 0; a < 3; ++a) {
+      const int v = e[1 + a];
+      inside = inside && v >= lo[a] && v <= hi[a];
+      on_face = on_face || v == lo[a] || v == hi[a];
+    }
+    if (inside && on_face) return false;
+  }
+  return true;
 }
 
 }  // namespace fdtd

@@ -8,11 +8,13 @@
 #include "ntff_post.hpp"
 #include "ntff_block.hpp"
 #include "multi_kernels.hpp"
+#include "e_tail.hpp"
 #include "earlystop_net.hpp"
 
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <climits>
 #include <cmath>
 #include <cstdarg>
@@ -81,6 +83,12 @@ struct FdtdHipContext {
   std::vector<void *> small_allocs;   // port tables, traces
   PortState ports{};
   std::vector<int> probe_host;        // [npr][PROBE_INTS], uploaded with the traces
+  // E elements (component 0..2 = Ex..Ez, i, j, k) the port epilogue writes (sources)
+  // and reads (voltage observers, E probes): whether the E-phase tail may run as one
+  // launch depends on them.
+  std::vector<ECell> tail_e_write, tail_e_read;
+  int e_tail_ok = 0;                  // e_tail_mergeable, as of the current run
+  int e_tail_merged = 0;              // the last step ran the E-phase tail as one launch
   int traces_ready = 0;
 
   int nff = 0;
@@ -107,7 +115,7 @@ struct FdtdHipContext {
   int use_graphs = 1;
   int *d_step = nullptr;
   hipGraphExec_t graph = nullptr;
-  int graph_key[7] = {-1, -1, -1, -1, -1, -1, -1};
+  int graph_key[8] = {-1, -1, -1, -1, -1, -1, -1, -1};
 
   FdtdHipTuning tune{0, 0, 0, 0};  // requested launch shape / coefficient storage (0 = default)
   int coef_mode = -1;           // CoefMode in use; -1 = not decided yet
@@ -151,6 +159,11 @@ struct FdtdHipContext {
     long long plane = (long long)t[1] * t[2];
     long long i = f / plane, r = f - i * plane, j = r / t[2], k = r - j * t[2];
     return i * SI + j * SJ + k;
+  }
+  // (component, i, j, k) of device offset `off` of E component `comp` (0..2).
+  ECell e_cell(int comp, long long off) const {
+    const int i = (int)(off / SI), r = (int)(off - (long long)i * SI), j = r / SJ;
+    return {comp, i, j, r - j * SJ};
   }
   bool fp32() const { return d.precision == FDTD_HIP_FP32; }
 };
@@ -493,6 +506,14 @@ int set_cpml_tiles(FdtdHipContext *c) {
   return FDTD_HIP_OK;
 }
 
+// Whether the step's E-phase tail may run as one launch (k_e_tail).  The NTFF reads
+// E only on the surface of its box [LI,UI] x [LJ,UJ] x [LK,UK] (0-based planes).
+bool e_tail_mergeable(const FdtdHipContext *c) {
+  const FarfieldState &f = c->ff;
+  const int lo[3] = {f.LI, f.LJ, f.LK}, hi[3] = {f.UI, f.UJ, f.UK};
+  return e_tail_disjoint(cpml_layout(c), c->tail_e_write, c->tail_e_read, c->nff > 0, lo, hi);
+}
+
 bool ntff_blocked(const FdtdHipContext *c) {
   if (c->nff == 0) return false;
   return c->ntff_mode == FDTD_HIP_NTFF_BLOCKED || (c->ntff_mode == FDTD_HIP_NTFF_AUTO && c->fp32());
@@ -570,34 +591,55 @@ int launch_step(FdtdHipContext *c, int ts, const int *tsp = nullptr) {
   if (t) HIP_TRY(hipEventRecord(c->ev[2], c->stream));
   launch_e<T>(c, g, me, ghost, fs);
   if (t) HIP_TRY(hipEventRecord(c->ev[3], c->stream));
-  if (c->ncpml[1])
-    hipLaunchKernelGGL(k_cpml_e<T>, dim3((unsigned)c->ncpml[1]), block, 0, c->stream, g, fs, c->cpml_tiles[1]);
-  if (t) HIP_TRY(hipEventRecord(c->ev[4], c->stream));
-  if (c->ports.nvs + c->ports.nsv + c->ports.nsi + c->ports.npr > 0) {
-    PortState ps = c->ports;
-    ps.ndft = c->ndft; ps.w = c->dft_w; ps.dft = c->dft_acc;
-    hipLaunchKernelGGL(k_port_epilogue<T>, dim3(1), block, 0, c->stream, g, ps, tsp, ts);
+  // The E-phase tail: E CPML, port epilogue (source injection, captures, running port
+  // DFT), NTFF.  One launch when e_tail_mergeable holds and per-kernel timing is off.
+  const bool ports = c->ports.nvs + c->ports.nsv + c->ports.nsi + c->ports.npr > 0;
+  PortState ps = c->ports;
+  ps.ndft = c->ndft; ps.w = c->dft_w; ps.dft = c->dft_acc;
+  const long total =
+      c->nff > 0 ? 2 * ((long)c->ff.nj * c->ff.nk + (long)c->ff.ni * c->ff.nk + (long)c->ff.ni * c->ff.nj) : 0;
+  const int nkind = c->nff > 0 ? (ntff_blocked(c) ? 2 : 1) : 0;
+  int row = ts;
+  if (nkind == 2 && !tsp) {
+    // Steps must follow on from the block; a jump (a new run) starts a new block.
+    if (ts != c->blk_t0 + c->blk_rows) {
+      if (ntff_flush<T>(c)) return FDTD_HIP_ERR;
+      c->blk_t0 = ts;
+      c->blk_rows = c->blk_flushed = 0;
+    }
+    row = c->blk_rows;
   }
-  if (t) HIP_TRY(hipEventRecord(c->ev[5], c->stream));
-  if (c->nff > 0) {
-    const long total = 2 * ((long)c->ff.nj * c->ff.nk + (long)c->ff.ni * c->ff.nk + (long)c->ff.ni * c->ff.nj);
-    if (ntff_blocked(c) && tsp) {
-      hipLaunchKernelGGL(k_ntff_gather<T>, dim3((unsigned)((total + 255) / 256)), block, 0, c->stream, g, c->ff,
-                         ntff_block<T>(c), ts);
+  const NtffBlock<T> nb = nkind == 2 ? ntff_block<T>(c) : NtffBlock<T>{};
+  c->e_tail_merged = !t && c->e_tail_ok;
+  if (c->e_tail_merged) {
+    const long nblk = c->ncpml[1] + (ports ? 1 : 0) + (total + 255) / 256;
+    const int nt = (int)c->ncpml[1], pp = ports ? 1 : 0;
+    if (nblk > 0) {
+      const dim3 grid((unsigned)nblk);
+      if (nkind == 2)
+        hipLaunchKernelGGL((k_e_tail<T, 2>), grid, block, 0, c->stream, g, fs, c->cpml_tiles[1], nt, pp, ps, c->ff, nb, row, tsp, ts);
+      else if (nkind == 1)
+        hipLaunchKernelGGL((k_e_tail<T, 1>), grid, block, 0, c->stream, g, fs, c->cpml_tiles[1], nt, pp, ps, c->ff, nb, row, tsp, ts);
+      else
+        hipLaunchKernelGGL((k_e_tail<T, 0>), grid, block, 0, c->stream, g, fs, c->cpml_tiles[1], nt, pp, ps, c->ff, nb, row, tsp, ts);
+    }
+  } else {
+    if (c->ncpml[1])
+      hipLaunchKernelGGL(k_cpml_e<T>, dim3((unsigned)c->ncpml[1]), block, 0, c->stream, g, fs, c->cpml_tiles[1]);
+    if (t) HIP_TRY(hipEventRecord(c->ev[4], c->stream));
+    if (ports) hipLaunchKernelGGL(k_port_epilogue<T>, dim3(1), block, 0, c->stream, g, ps, tsp, ts);
+    if (t) HIP_TRY(hipEventRecord(c->ev[5], c->stream));
+    if (nkind == 2)
+      hipLaunchKernelGGL(k_ntff_gather<T>, dim3((unsigned)((total + 255) / 256)), block, 0, c->stream, g, c->ff, nb, row);
+    else if (nkind == 1)
+      hipLaunchKernelGGL(k_ntff_direct<T>, dim3((unsigned)((total + 255) / 256)), block, 0, c->stream, g, c->ff, tsp, ts);
+  }
+  if (nkind == 2) {
+    if (tsp) {
       if (ts == NTFF_K - 1) ntff_fold<T>(c, tsp, 0, 0, NTFF_K);
-    } else if (ntff_blocked(c)) {
-      // Steps must follow on from the block; a jump (a new run) starts a new block.
-      if (ts != c->blk_t0 + c->blk_rows) {
-        if (ntff_flush<T>(c)) return FDTD_HIP_ERR;
-        c->blk_t0 = ts;
-        c->blk_rows = c->blk_flushed = 0;
-      }
-      hipLaunchKernelGGL(k_ntff_gather<T>, dim3((unsigned)((total + 255) / 256)), block, 0, c->stream, g, c->ff,
-                         ntff_block<T>(c), c->blk_rows);
+    } else {
       ++c->blk_rows;
       if (c->blk_rows == NTFF_K && ntff_flush<T>(c)) return FDTD_HIP_ERR;
-    } else {
-      hipLaunchKernelGGL(k_ntff_direct<T>, dim3((unsigned)((total + 255) / 256)), block, 0, c->stream, g, c->ff, tsp, ts);
     }
   }
   if (t) HIP_TRY(hipEventRecord(c->ev[6], c->stream));
@@ -619,10 +661,11 @@ int launch_step(FdtdHipContext *c, int ts, const int *tsp = nullptr) {
 constexpr int GRAPH_STEPS = NTFF_K;
 
 // Everything a captured step depends on besides the device data itself.
-void graph_key(const FdtdHipContext *c, int key[7]) {
+void graph_key(const FdtdHipContext *c, int key[8]) {
   key[0] = c->tune.planes_per_workgroup; key[1] = c->tune.cells_per_thread;
   key[2] = c->tune.coefficient_storage; key[3] = c->coef_mode;
   key[4] = ntff_blocked(c) ? 1 : 0; key[5] = c->ndft; key[6] = c->tune.fuse_cpml;
+  key[7] = c->e_tail_ok;
 }
 
 // Capture GRAPH_STEPS steps plus the step advance into an executable graph.
@@ -653,7 +696,7 @@ int run_steps(FdtdHipContext *c, int ts, int count) {
     const int t = ts + s;
     const bool aligned = !ntff_blocked(c) || (c->blk_rows == 0 && c->blk_t0 == t);
     if (c->use_graphs && !c->timing && count - s >= GRAPH_STEPS && aligned) {
-      int key[7];
+      int key[8];
       graph_key(c, key);
       if (!c->graph || std::memcmp(key, c->graph_key, sizeof key) != 0) {
         if (build_graph<T>(c)) return FDTD_HIP_ERR;
@@ -818,6 +861,7 @@ int fdtd_hip_upload_vsource(FdtdHipContext *c, int dir, const int *idx, const do
       if (idx[m] == idx[n]) return fail("upload_vsource: duplicate edge index %d", idx[n]);
     di[n] = (int)c->to_dev(comp, idx[n]);
   }
+  for (int n = 0; n < count; ++n) c->tail_e_write.push_back(c->e_cell(dir, di[n]));
   VSourceDev &v = c->ports.vs[c->ports.nvs];
   v.dir = dir; v.count = count;
   int *pi = nullptr; double *pc = nullptr, *pw = nullptr;
@@ -842,6 +886,7 @@ int fdtd_hip_upload_svoltage(FdtdHipContext *c, int dir, const int *idx, int cou
     if (idx[n] < 0 || idx[n] >= c->comp_len(comp)) return fail("upload_svoltage: edge index %d outside E%c", idx[n], "xyz"[dir]);
     di[n] = (int)c->to_dev(comp, idx[n]);
   }
+  for (int n = 0; n < count; ++n) c->tail_e_read.push_back(c->e_cell(dir, di[n]));
   SVoltageDev &s = c->ports.sv[c->ports.nsv];
   int *pi = nullptr;
   if (upload_small(c, di, &pi)) return FDTD_HIP_ERR;
@@ -925,6 +970,7 @@ int fdtd_hip_run_batch_async(FdtdHipContext *c, int tsStart, int count) {
   if (ensure_traces(c) || finalize_coefficients(c)) return FDTD_HIP_ERR;
   if (set_cpml_tiles(c)) return FDTD_HIP_ERR;
   if (ntff_blocked(c) && ensure_ntff_buffer(c)) return FDTD_HIP_ERR;
+  c->e_tail_ok = e_tail_mergeable(c) ? 1 : 0;
   if (c->fp32() ? run_steps<float>(c, tsStart, count) : run_steps<double>(c, tsStart, count)) return FDTD_HIP_ERR;
   // Leave the accumulators complete at the end of every batch.
   if (ntff_blocked(c) && (c->fp32() ? ntff_flush<float>(c) : ntff_flush<double>(c))) return FDTD_HIP_ERR;
@@ -974,6 +1020,7 @@ int fdtd_hip_upload_field_probe(FdtdHipContext *c, int magnetic, int component, 
         return fail("upload_field_probe: %c%c(%d,%d,%d) outside its %dx%dx%d array", magnetic ? 'H' : 'E',
                     "xyz"[cc], ii, jj, kk, dims[0], dims[1], dims[2]);
       d[4 + 4 * cc + t] = (int)c->to_dev(comp, ((long long)ii * dims[1] + jj) * dims[2] + kk);
+      if (!magnetic) c->tail_e_read.push_back(c->e_cell(cc, d[4 + 4 * cc + t]));
     }
   }
   c->probe_host.insert(c->probe_host.end(), d, d + PROBE_INTS);
@@ -996,6 +1043,12 @@ int fdtd_hip_read_probes(FdtdHipContext *c, int tsStart, int count, double *out)
 int fdtd_hip_run_batch(FdtdHipContext *c, int tsStart, int count, double *svOut, double *siOut) {
   if (fdtd_hip_run_batch_async(c, tsStart, count)) return FDTD_HIP_ERR;
   return fdtd_hip_read_traces(c, tsStart, count, svOut, siOut);
+}
+
+int fdtd_hip_get_step_info(FdtdHipContext *c, int *e_tail_merged) {
+  if (!c || !e_tail_merged) return fail("get_step_info: null argument");
+  *e_tail_merged = c->e_tail_merged;
+  return FDTD_HIP_OK;
 }
 
 int fdtd_hip_query(FdtdHipContext *c, int *idle) {

@@ -50,7 +50,7 @@ EXPORTS = [
     "fdtd_hip_upload_svoltage", "fdtd_hip_upload_scurrent", "fdtd_hip_upload_field_probe",
     "fdtd_hip_read_probes", "fdtd_hip_upload_farfield",
     "fdtd_hip_upload_port_dft", "fdtd_hip_run_batch", "fdtd_hip_run_batch_async", "fdtd_hip_read_traces",
-    "fdtd_hip_query", "fdtd_hip_read_field", "fdtd_hip_write_field",
+    "fdtd_hip_query", "fdtd_hip_get_step_info", "fdtd_hip_read_field", "fdtd_hip_write_field",
     "fdtd_hip_read_psi", "fdtd_hip_write_psi", "fdtd_hip_read_farfield", "fdtd_hip_read_port_dft", "fdtd_hip_radiated_power", "fdtd_hip_directivity",
     "fdtd_hip_set_ntff_mode", "fdtd_hip_get_ntff_mode", "fdtd_hip_set_graphs",
     "fdtd_hip_set_tuning", "fdtd_hip_get_tuning", "fdtd_hip_autotune",
@@ -105,6 +105,7 @@ def lib():
         L.fdtd_hip_run_batch_async.argtypes = [P, I, I]
         L.fdtd_hip_read_traces.argtypes = [P, I, I, D, D]
         L.fdtd_hip_query.argtypes = [P, IP]
+        L.fdtd_hip_get_step_info.argtypes = [P, IP]
         L.fdtd_hip_read_field.argtypes = [P, I, D, LL]
         L.fdtd_hip_write_field.argtypes = [P, I, D, LL]
         L.fdtd_hip_read_psi.argtypes = [P, I, I, D, LL]

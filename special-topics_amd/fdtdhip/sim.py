@@ -258,6 +258,12 @@ class HipSim:
         native.check(self.L.fdtd_hip_get_kernel_times(self.h, C.byref(t)), "get_kernel_times")
         return list(t.ms), list(t.launches)
 
+    def e_tail_merged(self) -> bool:
+        """Whether the last step ran E CPML, port epilogue and NTFF as one launch."""
+        v = C.c_int(0)
+        native.check(self.L.fdtd_hip_get_step_info(self.h, C.byref(v)), "get_step_info")
+        return bool(v.value)
+
 
 class HipMulti:
     """Several HipSim contexts stepped together, one launch per phase over all of them

@@ -48,6 +48,13 @@ static int check(const CpmlLayout &L, const char *name) {
                         faces_of(L, ph, co));
           if (seen[i * plane + (long)j * (nz + 1) + k]++ && bad++ < 5)
             std::printf("  %s mode %d phase %d: (%d,%d,%d) visited twice\n", name, mode, ph, i, j, k);
+          // The E pass changes component c where its tile mask holds a face of one of c's
+          // transverse axes; cpml_e_touches must say so exactly.
+          for (int comp = 0; ph == 1 && mode == CPML_SEPARATE && comp < 3; ++comp) {
+            const bool term = (mask & ~(3u << (2 * comp))) != 0;
+            if (term != cpml_e_touches(L, comp, i, j, k) && bad++ < 5)
+              std::printf("  %s: cpml_e_touches(E%c, %d,%d,%d) = %d\n", name, "xyz"[comp], i, j, k, !term);
+          }
         }
       }
       const int i_end = ph == 0 ? nx + 1 : nx;
@@ -60,6 +67,10 @@ static int check(const CpmlLayout &L, const char *name) {
             if (mode == CPML_FUSE_ALL) owned = false;
             if (mode == CPML_FUSE_Z && (m & (3u << 4)) && i < i_end) owned = false;
             cells += owned;
+            if (ph == 1 && !m)
+              for (int comp = 0; comp < 3; ++comp)
+                if (cpml_e_touches(L, comp, i, j, k) && bad++ < 5)
+                  std::printf("  %s: cpml_e_touches(E%c, %d,%d,%d) outside every slab\n", name, "xyz"[comp], i, j, k);
             if (owned != (seen[i * plane + (long)j * (nz + 1) + k] != 0) && bad++ < 5)
               std::printf("  %s mode %d phase %d: (%d,%d,%d) owned=%d but visited=%d\n", name, mode, ph, i, j, k, owned,
                           seen[i * plane + (long)j * (nz + 1) + k]);
@@ -85,8 +96,36 @@ static CpmlLayout standard(int nx, int ny, int nz, int npml, unsigned absent) {
   return L;
 }
 
-int main() {
+// e_tail_disjoint on constructed cases, one per condition.
+static int check_e_tail() {
+  const CpmlLayout L = standard(40, 40, 40, 6, 0);     // E slabs [1,7) and [34,40) on every axis
+  const int lo[3] = {10, 10, 10}, hi[3] = {30, 30, 30};
+  struct Case { const char *what; std::vector<ECell> w, r; bool ntff; int lo0; bool want; };
+  const Case cases[] = {
+      {"interior source and observer", {{2, 20, 20, 20}}, {{2, 20, 20, 21}}, true, 10, true},
+      {"no NTFF, interior ports", {{0, 20, 20, 20}}, {}, false, 10, true},
+      {"source in a z slab (Ex has z terms)", {{0, 20, 20, 3}}, {}, false, 10, false},
+      {"Ez in an x slab", {{2, 36, 20, 20}}, {}, false, 10, false},
+      {"Ez in a z slab only (no z terms)", {{2, 20, 20, 3}}, {}, false, 10, true},
+      {"observer in a y slab", {}, {{0, 20, 38, 20}}, false, 10, false},
+      {"NTFF box reaching an x slab", {{2, 20, 20, 20}}, {}, true, 6, false},
+      {"source on the NTFF surface", {{1, 10, 20, 20}}, {}, true, 10, false},
+      {"source on a surface edge", {{1, 30, 30, 20}}, {}, true, 10, false},
+      {"source outside the NTFF box", {{1, 32, 20, 20}}, {}, true, 10, true},
+      {"observer on the NTFF surface (reads only)", {}, {{1, 10, 20, 20}}, true, 10, true},
+  };
   int bad = 0;
+  for (const Case &c : cases) {
+    const int l[3] = {c.lo0, lo[1], lo[2]};
+    const bool got = e_tail_disjoint(L, c.w, c.r, c.ntff, l, hi);
+    std::printf("e_tail_disjoint: %s: %d\n", c.what, got);
+    if (got != c.want) { std::printf("  expected %d\n", c.want); ++bad; }
+  }
+  return bad;
+}
+
+int main() {
+  int bad = check_e_tail();
   bad += check(standard(64, 72, 80, 8, 0), "64x72x80 npml 8");
   bad += check(standard(39, 136, 136, 8, 0), "39x136x136 npml 8");
   bad += check(standard(21, 17, 70, 4, 1u << 3), "21x17x70 npml 4 no yp");
