@@ -612,16 +612,16 @@ int launch_step(FdtdHipContext *c, int ts, const int *tsp = nullptr) {
   const NtffBlock<T> nb = nkind == 2 ? ntff_block<T>(c) : NtffBlock<T>{};
   c->e_tail_merged = !t && c->e_tail_ok;
   if (c->e_tail_merged) {
-    const long nblk = c->ncpml[1] + (ports ? 1 : 0) + (total + 255) / 256;
-    const int nt = (int)c->ncpml[1], pp = ports ? 1 : 0;
+    const int nf = (int)((total + 255) / 256), pp = ports ? 1 : 0;
+    const long nblk = c->ncpml[1] + pp + nf;
     if (nblk > 0) {
       const dim3 grid((unsigned)nblk);
       if (nkind == 2)
-        hipLaunchKernelGGL((k_e_tail<T, 2>), grid, block, 0, c->stream, g, fs, c->cpml_tiles[1], nt, pp, ps, c->ff, nb, row, tsp, ts);
+        hipLaunchKernelGGL((k_e_tail<T, 2>), grid, block, 0, c->stream, g, fs, c->cpml_tiles[1], pp, nf, ps, c->ff, nb, row, tsp, ts);
       else if (nkind == 1)
-        hipLaunchKernelGGL((k_e_tail<T, 1>), grid, block, 0, c->stream, g, fs, c->cpml_tiles[1], nt, pp, ps, c->ff, nb, row, tsp, ts);
+        hipLaunchKernelGGL((k_e_tail<T, 1>), grid, block, 0, c->stream, g, fs, c->cpml_tiles[1], pp, nf, ps, c->ff, nb, row, tsp, ts);
       else
-        hipLaunchKernelGGL((k_e_tail<T, 0>), grid, block, 0, c->stream, g, fs, c->cpml_tiles[1], nt, pp, ps, c->ff, nb, row, tsp, ts);
+        hipLaunchKernelGGL((k_e_tail<T, 0>), grid, block, 0, c->stream, g, fs, c->cpml_tiles[1], pp, nf, ps, c->ff, nb, row, tsp, ts);
     }
   } else {
     if (c->ncpml[1])
