@@ -400,6 +400,7 @@ __device__ __forceinline__ void update_e_body(
   const bool jin = j >= 1 && j < ny;
   int c = i0 * SI + p;
   V hz_m = ldv<T, W>(Hz + c - SI), hy_m = ldv<T, W>(Hy + c - SI);   // H at plane i-1
+  V ex0 = V{};     // Ex at plane 0 after its bulk update, before any CPML term (the wall reads it)
   for (int i = i0; i < i1; ++i, c += SI) {
     const bool iin = i >= 1 && i < nx;
     const V hz_i = ldv<T, W>(Hz + c), hy_i = ldv<T, W>(Hy + c), hx_i = ldv<T, W>(Hx + c);
@@ -410,7 +411,6 @@ __device__ __forceinline__ void update_e_body(
     coeffs<T, CM, W>(g.ct, tc, 3, 0, c, g.C[CEXE], g.C[CEXHZ], g.C[CEXHY], c0, c1, c2);
     coeffs<T, CM, W>(g.ct, tc, 3, 1, c, g.C[CEYE], g.C[CEYHX], g.C[CEYHZ], c3, c4, c5);
     coeffs<T, CM, W>(g.ct, tc, 3, 2, c, g.C[CEZE], g.C[CEZHY], g.C[CEZHX], c6, c7, c8);
-    V exb = ex;
     // The six H differences of the curls, shared with the CPML terms (H does not change
     // during the E phase).
     V dy_ex, dz_ex, dz_ey, dx_ey, dx_ez, dy_ez;
@@ -430,7 +430,6 @@ __device__ __forceinline__ void update_e_body(
         bool t = jin && kin;
         T v = ex[e];
         if (t) v = c0[e] * v + c1[e] * dy_ex[e] + c2[e] * dz_ex[e];
-        exb[e] = v;
         ex[e] = v;
       }
       if (j < ny && k <= nz) {                             // Ey (nx+1, ny, nz+1)
@@ -446,8 +445,9 @@ __device__ __forceinline__ void update_e_body(
         ez[e] = v;
       }
     }
+    if (i == 0) ex0 = ex;
     if (FC != CPML_SEPARATE) {
-      // As in update_h_body; exb keeps the pre-CPML Ex that the ghost plane reads.
+      // As in update_h_body; ex0 keeps the pre-CPML Ex that the ghost plane reads.
 #pragma unroll
       for (int e = 0; e < W; ++e) {
         const int k = k0 + e;
@@ -467,44 +467,48 @@ __device__ __forceinline__ void update_e_body(
     stv_stream<T, W>(Ey + c, ey);
     stv_stream<T, W>(Ez + c, ez);
 
-    if (ghost && i == 0) {
-      const int cg = nx * SI + p;
-#pragma unroll
-      for (int e = 0; e < W; ++e) {
-        const int k = k0 + e;
-        const bool kin = k >= 1 && k < nz;
-        if (j < ny && kin) {                               // Ey(nx, j, k)
-          const T dHx = Hx[cg + e] - Hx[cg + e - 1];
-          const T dHz = exb[e] - Hz[cg + e - SI];
-          T v = g.C[CEYE][cg + e] * Ey[cg + e] + g.C[CEYHX][cg + e] * dHx + g.C[CEYHZ][cg + e] * dHz;
-          Ey[cg + e] = v;
-        }
-        if (jin && k < nz) {                               // Ez(nx, j, k)
-          const int f = j * nz + k;                        // reference flat index into Hz
-          const int fr = f / (nz + 1);
-          const T dHy = Hz[fr * SJ + (f - fr * (nz + 1))] - Hy[cg + e - SI];
-          const T dHx = Hx[cg + e] - Hx[cg + e - SJ];
-          T v = g.C[CEZE][cg + e] * Ez[cg + e] + g.C[CEZHY][cg + e] * dHy + g.C[CEZHX][cg + e] * dHx;
-          Ez[cg + e] = v;
-        }
-      }
-    }
-    if (FC == CPML_FUSE_ALL && i == 0) {
-      const int cg = nx * SI + p;
-#pragma unroll
-      for (int e = 0; e < W; ++e) {
-        const int k = k0 + e, ce = cg + e;
-        if (k > nz || !fused_cpml_cell<T, FC, false>(fs, nx, j, k)) continue;
-        if (j < ny)                                        // Ey(nx): x then z faces
-          Ey[ce] = cpml_terms<T, 0, 2, 2, 3, false>(fs, Ey[ce], nx, j, k, ny, nz + 1,
-              [&] { return Hz[ce] - Hz[ce - SI]; }, [&] { return Hx[ce] - Hx[ce - 1]; });
-        if (k < nz)                                        // Ez(nx): x then y faces
-          Ez[ce] = cpml_terms<T, 0, 3, 1, 2, false>(fs, Ez[ce], nx, j, k, ny + 1, nz,
-              [&] { return Hy[ce] - Hy[ce - SI]; }, [&] { return Hx[ce] - Hx[ce - SJ]; });
-      }
-    }
     hz_m = hz_i;
     hy_m = hy_i;
+  }
+  // The i = nx wall, once per column by the chunk that holds plane 0.  Only this thread
+  // writes the wall's Ey/Ez in this phase, and H does not change, so the wall may follow
+  // the march; ex0 is Ex(0) as the bulk left it.  Kept out of the march loop, its
+  // registers do not add to the loop's.
+  if (ghost && i0 == 0) {
+    const int cg = nx * SI + p;
+#pragma unroll
+    for (int e = 0; e < W; ++e) {
+      const int k = k0 + e;
+      const bool kin = k >= 1 && k < nz;
+      if (j < ny && kin) {                               // Ey(nx, j, k)
+        const T dHx = Hx[cg + e] - Hx[cg + e - 1];
+        const T dHz = ex0[e] - Hz[cg + e - SI];
+        T v = g.C[CEYE][cg + e] * Ey[cg + e] + g.C[CEYHX][cg + e] * dHx + g.C[CEYHZ][cg + e] * dHz;
+        Ey[cg + e] = v;
+      }
+      if (jin && k < nz) {                               // Ez(nx, j, k)
+        const int f = j * nz + k;                        // reference flat index into Hz
+        const int fr = f / (nz + 1);
+        const T dHy = Hz[fr * SJ + (f - fr * (nz + 1))] - Hy[cg + e - SI];
+        const T dHx = Hx[cg + e] - Hx[cg + e - SJ];
+        T v = g.C[CEZE][cg + e] * Ez[cg + e] + g.C[CEZHY][cg + e] * dHy + g.C[CEZHX][cg + e] * dHx;
+        Ez[cg + e] = v;
+      }
+    }
+  }
+  if (FC == CPML_FUSE_ALL && i0 == 0) {
+    const int cg = nx * SI + p;
+#pragma unroll
+    for (int e = 0; e < W; ++e) {
+      const int k = k0 + e, ce = cg + e;
+      if (k > nz || !fused_cpml_cell<T, FC, false>(fs, nx, j, k)) continue;
+      if (j < ny)                                        // Ey(nx): x then z faces
+        Ey[ce] = cpml_terms<T, 0, 2, 2, 3, false>(fs, Ey[ce], nx, j, k, ny, nz + 1,
+            [&] { return Hz[ce] - Hz[ce - SI]; }, [&] { return Hx[ce] - Hx[ce - 1]; });
+      if (k < nz)                                        // Ez(nx): x then y faces
+        Ez[ce] = cpml_terms<T, 0, 3, 1, 2, false>(fs, Ez[ce], nx, j, k, ny + 1, nz,
+            [&] { return Hy[ce] - Hy[ce - SI]; }, [&] { return Hx[ce] - Hx[ce - SJ]; });
+    }
   }
 }
 
