@@ -303,7 +303,8 @@ def main():
 
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
-              "(the time loop is single-GPU; see DESIGN.md for how it would shard).", file=sys.stderr)
+              "(the time loop runs on one GPU; across GPUs it would split the grid into x-slabs that "
+              "exchange one plane of halo per phase).", file=sys.stderr)
         sys.exit(2)
 
     import torch

@@ -140,7 +140,7 @@ def gen_train(first, count, out, seed=4, workdir="/tmp/es-data", batch=64, jobs=
 
 
 def gen_heldout(first, count, out, seed=4, workdir="/tmp/es-heldout", batch=64, jobs=16):
-    """Per held-out variant: the full-length FP64 S11 (the judge) and the FP32 prefix
+    """Per held-out variant: the full-length FP64 S11 (the reference result) and the FP32 prefix
     traces at full rate (what an early-stopped run sees)."""
     paths, params = _scenarios(first, count, seed, workdir)
     b64 = LazyBuilds(paths, os.path.join(workdir, "f64"), fp32=False, jobs=jobs, ahead=2 * batch)
@@ -283,7 +283,7 @@ def time_to_solution(weights_path, first, count, seed=4, workdir="/tmp/es-tts"):
 
 def time_to_solution_batched(weights_path, first, count, seed=4, workdir="/tmp/es-tts-batch", jobs=16):
     """The same comparison with every variant in one batched launch on each side: the
-    FP64 judge runs all `count` variants for 7000 steps; the early-stopped path runs
+    FP64 reference runs all `count` variants for 7000 steps; the early-stopped path runs
     them in FP32 for 1500 steps, evaluates the network on the whole batch and forms
     S11.  A fallback costs its share (1/count) of the FP64 batch.  Builds, uploads and
     the batch autotuner run before each clock starts; times are per variant."""
