@@ -233,6 +233,10 @@ def sweep_study(workdir, count=64, batch=64, seed=4):
         "batch": batch, "batch_tuning": list(sweep.batch_tuning), "build_s": round(build_s, 2),
         "seconds": round(secs, 3),
         "mcell_updates_per_s": round(cells / secs / 1e6, 1), "ms_per_variant": round(secs / count * 1e3, 2),
+        # The batched time loops alone (launch to completion), without uploads, the batch
+        # autotuner, S11 or the far-field post.
+        "loop_s": round(sweep.loop_seconds, 3), "autotune_s": round(sweep.autotune_seconds, 3),
+        "loop_mcell_updates_per_s": round(cells / sweep.loop_seconds / 1e6, 1) if sweep.loop_seconds else None,
         "resonance_ghz_min_median_max": [round(float(x), 3) for x in (res_ghz.min(), np.median(res_ghz), res_ghz.max())],
         "s11_min_db_median": round(float(np.median(smin)), 2),
         "peak_directivity_dbi_median": round(float(np.median(dpk)), 2),

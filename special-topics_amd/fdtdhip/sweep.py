@@ -101,6 +101,8 @@ class Sweep:
         self.precision, self.boundary, self.inflight = precision, boundary, inflight
         self.batch = batch
         self.batch_tuning = None
+        self.loop_seconds = 0.0       # batched runs only: launch to completion of the time loops
+        self.autotune_seconds = 0.0
         self.tuning, self.keep_traces, self.farfield, self.steps = tuning, keep_traces, farfield, steps
         self.with_s11 = s11
         self.progress_every = progress_every
@@ -159,10 +161,13 @@ class Sweep:
             if self.tuning is not None:
                 self.batch_tuning = tuple(self.tuning) + (0,) * (4 - len(self.tuning))
             else:
+                ta = time.perf_counter()
                 m = HipMulti([j["sim"] for j in jobs])
                 self.batch_tuning = m.autotune(2)
                 m.close()
+                self.autotune_seconds += time.perf_counter() - ta
         multi = HipMulti([j["sim"] for j in jobs], *self.batch_tuning)
+        multi.t_launch = time.perf_counter()
         multi.run_async(0, steps.pop())
         return multi
 
@@ -187,6 +192,10 @@ class Sweep:
             for j in jobs:
                 cell_steps += int(np.prod(j["problem"].dims)) * j["steps"]
             following = group(nxt)            # uploads overlap the running group
+            multi.synchronize()
+            # Launch to completion; with several groups it also covers the next group's
+            # uploads whenever those take longer than the loop.
+            self.loop_seconds += time.perf_counter() - multi.t_launch
             for j in jobs:
                 results[j["idx"]] = self._finish(j)
             multi.close()
@@ -243,9 +252,11 @@ def main():
     a = ap.parse_args()
     tuning = tuple(int(x) for x in a.tuning.split(",")) if a.tuning else None
     files = build_all(a.scenarios, a.workdir, fp32=a.precision == "fp32", jobs=16)
-    res, secs, cells = Sweep(a.precision, a.boundary, a.inflight, tuning=tuning, keep_traces=False,
-                             steps=a.steps, batch=a.batch).run(files)
+    sw = Sweep(a.precision, a.boundary, a.inflight, tuning=tuning, keep_traces=False, steps=a.steps, batch=a.batch)
+    res, secs, cells = sw.run(files)
     summary = {"variants": len(res), "seconds": round(secs, 3), "mcell_updates_per_s": round(cells / secs / 1e6, 1),
+               "loop_s": round(sw.loop_seconds, 3),
+               "loop_mcell_updates_per_s": round(cells / sw.loop_seconds / 1e6, 1) if sw.loop_seconds else None,
                "results": [{k: v for k, v in r.items() if k not in ("s11", "s11_db", "cuts", "v", "i")} for r in res]}
     text = json.dumps(summary)
     if a.out:
