@@ -15,7 +15,6 @@
 
 #include <algorithm>
 #include <array>
-#include <climits>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
