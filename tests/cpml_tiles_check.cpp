@@ -131,6 +131,9 @@ int main() {
   bad += check(standard(21, 17, 70, 4, 1u << 3), "21x17x70 npml 4 no yp");
   bad += check(standard(9, 11, 13, 2, (1u << 0) | (1u << 5)), "9x11x13 npml 2 no xn, zp");
   bad += check(standard(12, 12, 12, 0, 63u), "12^3 no CPML");
+  // The largest axis the 16-bit tile coordinates hold (planes 0..n with n = 65534).
+  bad += check(standard(CPML_TILE_MAX_AXIS, 4, 5, 1, 0), "65534x4x5 npml 1");
+  bad += check(standard(4, 5, CPML_TILE_MAX_AXIS, 1, 0), "4x5x65534 npml 1");
   // Seeded random layouts within the bounds upload_cpml accepts (2 nc < n, H planes in
   // [0, n), E planes in [1, n)); slabs may sit anywhere, even overlap.
   std::mt19937 rng(7);
