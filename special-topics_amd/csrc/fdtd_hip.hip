@@ -1409,6 +1409,7 @@ struct FdtdHipMulti {
   int cm = -1;                  // coefficient storage shared by every variant
   int ntff = 0;                 // 0: no variant has NTFF, 1: direct, 2: blocked
   bool ports = false;
+  bool e_tail = false;          // every variant's E-phase tail may run as one launch
   // Device descriptors and workgroup maps, with the host images last uploaded.
   void *d_desc = nullptr;
   unsigned *d_map = nullptr;    // [H tiles | E tiles | H CPML | E CPML | NTFF blocks]
@@ -1486,7 +1487,9 @@ int multi_prepare(FdtdHipMulti *m) {
   m->cm = m->ctx[0]->coef_mode;
   m->ntff = 0;
   m->ports = false;
+  m->e_tail = true;
   for (FdtdHipContext *c : m->ctx) {
+    m->e_tail = m->e_tail && e_tail_mergeable(c);
     if (c->nff) {
       const int mode = ntff_blocked(c) ? 2 : 1;
       if (m->ntff && m->ntff != mode) return fail("multi: variants mix blocked and direct NTFF accumulation");
@@ -1543,6 +1546,7 @@ int multi_prepare(FdtdHipMulti *m) {
   map.push_back((unsigned)m->ntff);
   map.push_back(m->ports ? 1u : 0u);
   map.push_back((unsigned)m->tune.fuse_cpml);
+  map.push_back(m->e_tail ? 1u : 0u);
   const size_t dbytes = desc.size() * sizeof(VariantDesc<T>);
   const bool changed = dbytes != m->h_desc.size() || std::memcmp(desc.data(), m->h_desc.data(), dbytes) != 0 ||
                        map != m->h_map;
@@ -1613,7 +1617,7 @@ void multi_launch_phase(FdtdHipMulti *m, bool e_phase) {
   const VariantDesc<T> *vd = (const VariantDesc<T> *)m->d_desc;
   const int nb = e_phase ? m->nce : m->nch;
   const unsigned *map = m->d_map + m->nh + m->ne + (e_phase ? m->nch : 0);
-  if (nb == 0) return;
+  if (nb == 0 || (e_phase && m->e_tail)) return;     // the E CPML tiles then run in k_multi_e_tail
   if (e_phase) hipLaunchKernelGGL(k_multi_cpml_e<T>, dim3((unsigned)nb), dim3(256), 0, m->stream, vd, map);
   else hipLaunchKernelGGL(k_multi_cpml_h<T>, dim3((unsigned)nb), dim3(256), 0, m->stream, vd, map);
 }
@@ -1624,18 +1628,37 @@ int multi_step(FdtdHipMulti *m, int ts, const int *tsp = nullptr) {
   const VariantDesc<T> *vd = (const VariantDesc<T> *)m->d_desc;
   if (m->nh) multi_launch_phase<T>(m, false);
   if (m->ne) multi_launch_phase<T>(m, true);
-  if (m->ports)
-    hipLaunchKernelGGL(k_multi_port_epilogue<T>, dim3((unsigned)m->ctx.size()), dim3(256), 0, m->stream, vd, tsp, ts);
-  if (m->ntff == 1) {
-    hipLaunchKernelGGL(k_multi_ntff_direct<T>, dim3((unsigned)m->nfb), dim3(256), 0, m->stream, vd,
-                       m->d_map + m->nh + m->ne + m->nch + m->nce, tsp, ts);
-  } else if (m->ntff == 2) {
+  const unsigned *fmap = m->d_map + m->nh + m->ne + m->nch + m->nce;
+  int row = ts;
+  if (m->ntff == 2 && !tsp) {
     // Every variant's block state moves in lockstep (multi_run restarts them together).
-    FdtdHipContext *lead = nullptr;
-    for (FdtdHipContext *c : m->ctx) if (c->nff) { lead = c; break; }
-    const int row = tsp ? ts : lead->blk_rows;
-    hipLaunchKernelGGL(k_multi_ntff_gather<T>, dim3((unsigned)m->nfb), dim3(256), 0, m->stream, vd,
-                       m->d_map + m->nh + m->ne + m->nch + m->nce, row);
+    for (FdtdHipContext *c : m->ctx)
+      if (c->nff) { row = c->blk_rows; break; }
+  }
+  for (FdtdHipContext *c : m->ctx) c->e_tail_merged = m->e_tail ? 1 : 0;
+  if (m->e_tail) {
+    // E CPML tiles, port epilogues and NTFF blocks in one launch (k_multi_e_tail).
+    const int np = m->ports ? (int)m->ctx.size() : 0, nf = m->ntff ? m->nfb : 0;
+    const long nblk = (long)np + nf + m->nce;
+    const unsigned *cmap = m->d_map + m->nh + m->ne + m->nch;
+    if (nblk > 0) {
+      const dim3 grid((unsigned)nblk);
+      if (m->ntff == 2)
+        hipLaunchKernelGGL((k_multi_e_tail<T, 2>), grid, dim3(256), 0, m->stream, vd, np, fmap, nf, cmap, row, tsp, ts);
+      else if (m->ntff == 1)
+        hipLaunchKernelGGL((k_multi_e_tail<T, 1>), grid, dim3(256), 0, m->stream, vd, np, fmap, nf, cmap, row, tsp, ts);
+      else
+        hipLaunchKernelGGL((k_multi_e_tail<T, 0>), grid, dim3(256), 0, m->stream, vd, np, fmap, nf, cmap, row, tsp, ts);
+    }
+  } else {
+    if (m->ports)
+      hipLaunchKernelGGL(k_multi_port_epilogue<T>, dim3((unsigned)m->ctx.size()), dim3(256), 0, m->stream, vd, tsp, ts);
+    if (m->ntff == 1)
+      hipLaunchKernelGGL(k_multi_ntff_direct<T>, dim3((unsigned)m->nfb), dim3(256), 0, m->stream, vd, fmap, tsp, ts);
+    else if (m->ntff == 2)
+      hipLaunchKernelGGL(k_multi_ntff_gather<T>, dim3((unsigned)m->nfb), dim3(256), 0, m->stream, vd, fmap, row);
+  }
+  if (m->ntff == 2) {
     for (FdtdHipContext *c : m->ctx) {
       if (!c->nff) continue;
       if (tsp) {

@@ -160,4 +160,35 @@ __global__ void __launch_bounds__(256) k_multi_ntff_direct(const VariantDesc<T> 
   ntff_direct_body<T>(global_grid(d.g), d.ff, step_of(tsp, off), q);
 }
 
+// The batched E-phase tail in one launch, as k_e_tail for one problem: the port
+// epilogue blocks (one per variant), then the NTFF blocks (NTFF = 1 direct, 2 gather)
+// through their map, then the E CPML tiles through theirs.  Used only when every
+// variant's E CPML pass, epilogue and NTFF touch disjoint field elements; variants
+// never share memory.
+template <typename T, int NTFF>
+__global__ void __launch_bounds__(256) k_multi_e_tail(const VariantDesc<T> *__restrict__ vd, int nports,
+                                                      const unsigned *__restrict__ fmap, int nntff,
+                                                      const unsigned *__restrict__ cmap, int row,
+                                                      const int *tsp, int off) {
+  int b = (int)blockIdx.x;
+  if (b < nports) {
+    const VariantDesc<T> &d = vd[b];
+    if (d.has_ports) port_epilogue_body<T>(global_grid(d.g), d.ps, step_of(tsp, off));
+    return;
+  }
+  b -= nports;
+  if (b < nntff) {
+    const unsigned entry = fmap[b];
+    const VariantDesc<T> &d = vd[entry >> MULTI_LOCAL_BITS];
+    const long q = (long)(entry & MULTI_LOCAL_MASK) * blockDim.x + threadIdx.x;
+    if (NTFF == 1) ntff_direct_body<T>(global_grid(d.g), d.ff, step_of(tsp, off), q);
+    if (NTFF == 2) ntff_gather_body<T>(global_grid(d.g), d.ff, d.nb, row, q);
+    return;
+  }
+  const unsigned entry = cmap[b - nntff];
+  const VariantDesc<T> &d = vd[entry >> MULTI_LOCAL_BITS];
+  const CpmlTile t = as_global(d.te)[entry & MULTI_LOCAL_MASK];
+  cpml_e_tile<T>(global_grid(d.g), global_faces(d.fs), t, threadIdx.x);
+}
+
 }  // namespace fdtd
