@@ -286,6 +286,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-ifa", action="store_true", help="skip the IFA FP32-vs-FP64 time-to-solution study")
     ap.add_argument("--no-sweep", action="store_true", help="skip the 64-variant IFA sweep")
+    ap.add_argument("--sweep-batch", type=int, default=64,
+                    help="variants per batched launch in the sweep study (later groups upload while the previous one runs)")
     ap.add_argument("--no-early-stop", action="store_true", help="skip the early-stop time-to-solution study")
     ap.add_argument("--ntff-mode", default="auto", choices=["auto", "direct", "blocked"],
                     help="NTFF running DFT: step-by-step (direct) or K-blocked on the matrix cores")
@@ -428,7 +430,7 @@ def main():
 
     if not args.no_sweep:
         progress("64-variant IFA sweep")
-        line["sweep"] = sweep_study(args.workdir)
+        line["sweep"] = sweep_study(args.workdir, batch=args.sweep_batch)
 
     if not args.no_early_stop:
         progress("early-stop time-to-solution")
