@@ -28,6 +28,7 @@ import time
 import numpy as np
 
 from .problem import build_problem, read_problem
+from . import native
 from .sim import HipMulti, HipSim
 
 
@@ -97,9 +98,13 @@ class Sweep:
     order."""
 
     def __init__(self, precision="fp32", boundary="matlab", inflight=4, tuning=None, keep_traces=True,
-                 farfield=True, steps=None, s11=True, progress_every=0, batch=0):
+                 farfield=True, steps=None, s11=True, progress_every=0, batch=0, host_threads=8):
         self.precision, self.boundary, self.inflight = precision, boundary, inflight
         self.batch = batch
+        # Batched mode: contexts of a group are created (uploads) and finished (traces,
+        # S11, far field) by this many host threads; the library's calls release the GIL
+        # and each context has its own stream.
+        self.host_threads = max(1, host_threads)
         self.batch_tuning = None
         self.loop_seconds = 0.0       # batched runs only: launch to completion of the time loops
         self.autotune_seconds = 0.0
@@ -176,14 +181,16 @@ class Sweep:
         cell_steps = 0
 
         def group(first):
-            jobs = []
-            for k in range(first, min(first + self.batch, len(items))):
-                p, path = self._load(items, lazy, k)
-                jobs.append(self._start(k, p, launch=False))
+            ks = list(range(first, min(first + self.batch, len(items))))
+            loaded = [self._load(items, lazy, k) for k in ks]
+            jobs = list(pool.map(lambda kp: self._start(kp[0], kp[1][0], launch=False), zip(ks, loaded)))
+            for p, path in loaded:
                 if path and lazy:
                     os.remove(path)
             return jobs
 
+        native.lib()                          # load once, before the worker threads call it
+        pool = cf.ThreadPoolExecutor(max_workers=self.host_threads)
         nxt = 0
         jobs = group(nxt)
         while jobs:
@@ -196,12 +203,13 @@ class Sweep:
             # Launch to completion; with several groups it also covers the next group's
             # uploads whenever those take longer than the loop.
             self.loop_seconds += time.perf_counter() - multi.t_launch
-            for j in jobs:
-                results[j["idx"]] = self._finish(j)
+            for j, r in zip(jobs, pool.map(self._finish, jobs)):
+                results[j["idx"]] = r
             multi.close()
             if self.progress_every:
                 print(f"[sweep] {nxt}/{len(items)} done, {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
             jobs = following
+        pool.shutdown()
         return results, time.perf_counter() - t0, cell_steps
 
     def run(self, problems):
