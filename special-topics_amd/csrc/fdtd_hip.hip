@@ -15,11 +15,13 @@
 
 #include <algorithm>
 #include <array>
+#include <atomic>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <unordered_map>
 #include <vector>
 
@@ -1462,10 +1464,41 @@ void put_map(std::vector<unsigned> &map, int v, long n) {
   for (long q = 0; q < n; ++q) map.push_back(((unsigned)v << MULTI_LOCAL_BITS) | (unsigned)q);
 }
 
+// Build the deduplicated coefficient tables of every variant that still lacks them,
+// on up to 8 host threads: the work is host hashing plus uploads on each context's own
+// stream (about 10 ms per IFA variant, which one thread would run back to back).
+// A failure's message comes back to the calling thread.
+int build_tables_threaded(const std::vector<FdtdHipContext *> &ctx, bool indexed) {
+  if (!indexed) return FDTD_HIP_OK;
+  std::vector<FdtdHipContext *> todo;
+  for (FdtdHipContext *c : ctx)
+    if (!c->coef_built) todo.push_back(c);
+  if (todo.size() < 2) return FDTD_HIP_OK;
+  std::atomic<size_t> next{0};
+  std::vector<int> rc(todo.size(), FDTD_HIP_OK);
+  std::vector<std::string> msg(todo.size());
+  auto worker = [&] {
+    for (size_t k; (k = next.fetch_add(1)) < todo.size();) {
+      FdtdHipContext *c = todo[k];
+      if (hipSetDevice(c->device) != hipSuccess) { rc[k] = FDTD_HIP_ERR; msg[k] = "hipSetDevice failed"; continue; }
+      rc[k] = build_coefficient_tables(c);
+      if (rc[k]) msg[k] = g_last_error;
+    }
+  };
+  const size_t nth = std::min<size_t>(todo.size(), 8);
+  std::vector<std::thread> pool;
+  for (size_t t = 0; t < nth; ++t) pool.emplace_back(worker);
+  for (auto &t : pool) t.join();
+  for (size_t k = 0; k < todo.size(); ++k)
+    if (rc[k]) return fail("%s", msg[k].c_str());
+  return FDTD_HIP_OK;
+}
+
 // Apply the batch tuning to every context, make the coefficient storage uniform,
 // and (re)upload the descriptors and maps when they changed.
 template <typename T>
 int multi_prepare(FdtdHipMulti *m) {
+  if (build_tables_threaded(m->ctx, m->tune.coefficient_storage != 1)) return FDTD_HIP_ERR;
   for (int pass = 0; pass < 2; ++pass) {
     for (FdtdHipContext *c : m->ctx) {
       FdtdHipTuning t = m->tune;
