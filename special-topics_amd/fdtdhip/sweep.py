@@ -256,11 +256,13 @@ def main():
     ap.add_argument("--tuning", default=None,
                     help="planes,cells,storage,cpml for every launch (default: the batch autotuner)")
     ap.add_argument("--steps", type=int, default=None, help="steps per variant (default: the scenario's)")
+    ap.add_argument("--threads", type=int, default=8, help="host threads creating and finishing a batch's contexts")
     ap.add_argument("--out", default=None)
     a = ap.parse_args()
     tuning = tuple(int(x) for x in a.tuning.split(",")) if a.tuning else None
     files = build_all(a.scenarios, a.workdir, fp32=a.precision == "fp32", jobs=16)
-    sw = Sweep(a.precision, a.boundary, a.inflight, tuning=tuning, keep_traces=False, steps=a.steps, batch=a.batch)
+    sw = Sweep(a.precision, a.boundary, a.inflight, tuning=tuning, keep_traces=False, steps=a.steps, batch=a.batch,
+               host_threads=a.threads)
     res, secs, cells = sw.run(files)
     summary = {"variants": len(res), "seconds": round(secs, 3), "mcell_updates_per_s": round(cells / secs / 1e6, 1),
                "loop_s": round(sw.loop_seconds, 3),
