@@ -1481,8 +1481,13 @@ int build_tables_threaded(const std::vector<FdtdHipContext *> &ctx, bool indexed
     for (size_t k; (k = next.fetch_add(1)) < todo.size();) {
       FdtdHipContext *c = todo[k];
       if (hipSetDevice(c->device) != hipSuccess) { rc[k] = FDTD_HIP_ERR; msg[k] = "hipSetDevice failed"; continue; }
-      rc[k] = build_coefficient_tables(c);
-      if (rc[k]) msg[k] = g_last_error;
+      try {
+        rc[k] = build_coefficient_tables(c);
+        if (rc[k]) msg[k] = g_last_error;
+      } catch (const std::exception &e) {     // e.g. out of host memory: report, do not terminate
+        rc[k] = FDTD_HIP_ERR;
+        msg[k] = std::string("coefficient tables: ") + e.what();
+      }
     }
   };
   const size_t nth = std::min<size_t>(todo.size(), 8);
