@@ -123,7 +123,9 @@ __global__ void __launch_bounds__(256) k_ntff_mfma(FarfieldState fs, NtffBlock<T
 #pragma unroll
     for (int t = 0; t < NT; ++t) acc[t] = mfma_d4{0.0, 0.0, 0.0, 0.0};
     const T *col = nb.buf + s0 + (lane & 15);
-#pragma unroll 4
+    // Fully unrolled, so all 16 sample loads of the tile are issued before the first
+    // matrix-core step; the fold is HBM-bound and needs the loads in flight together.
+#pragma unroll
     for (int kk = 0; kk < NTFF_K; kk += 4) {
       const double b = (double)col[(long)(kk + kl) * nb.spad];
 #pragma unroll
