@@ -1878,8 +1878,10 @@ int fdtd_hip_multi_autotune(FdtdHipMulti *m, int steps_per_candidate, FdtdHipTun
     }
   hipEvent_t e0 = nullptr, e1 = nullptr;
   HIP_TRY(hipSetDevice(m->device));
-  HIP_TRY(hipEventCreate(&e0));
-  HIP_TRY(hipEventCreate(&e1));
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+    if (e0) (void)hipEventDestroy(e0);
+    return fail("multi_autotune: event creation failed");
+  }
   double best = 1e300;
   FdtdHipTuning best_t = m->tune;
   int rc = FDTD_HIP_OK;
