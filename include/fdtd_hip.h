@@ -259,8 +259,10 @@ void fdtd_hip_destroy(FdtdHipContext *ctx);
  * the reference, which runs one problem per process (the MATLAB sweep scripts loop
  * over scenarios).
  *
- * The contexts are borrowed, not owned: they must share a device and precision and
- * must not be run on their own while a batched run is in flight.  Tuning as in
+ * The contexts are borrowed, not owned: they must share a device and precision, must
+ * not be run on their own while a batched run is in flight, and must outlive the batch
+ * (fdtd_hip_multi_destroy first, then fdtd_hip_destroy; the Node and Python bindings
+ * enforce this order).  Tuning as in
  * FdtdHipTuning, one choice for the whole batch (planes per workgroup 0 = each
  * variant's default); with fuse_cpml 0 or 1 the CPML cells left to the separate pass
  * are covered by one more batched launch per phase.  When the variants' deduplicated

@@ -25,7 +25,7 @@
  *   radiatedPower(h, Float64Array out[nFreq])
  *   directivity(h, freqHz, radPower|null, theta, phi, dataTheta, dataPhi)
  *   setTiming(h, bool); kernelTimes(h) -> {ms:[6], launches:[6]}
- *   destroy(h)
+ *   destroy(h)                     (throws while a live batch holds h)
  *   readTraces(h, tsStart, count, Float64Array|null sv, Float64Array|null si)
  *   uploadFieldProbe(h, magnetic, component, i, j, k); readProbes(h, tsStart, count, Float64Array)
  *   multiCreate([h, ...], tuning|null) -> batch handle   (fdtd_hip_multi_*: many
@@ -37,6 +37,7 @@
  */
 #define NAPI_VERSION 8
 #include <node_api.h>
+#include <stdint.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -131,24 +132,54 @@ static int prop_double(napi_env env, napi_value obj, const char *key, double *ou
 /* A handle is an external around a HandleBox.  destroy()/multiDestroy() free the
  * library object and clear the box, so a second call on the same handle throws
  * instead of touching freed memory; the box itself goes with the JS value, and a
- * handle collected without destroy() frees its library object then. */
-typedef struct {
+ * handle collected without destroy() frees its library object then.
+ *
+ * A batch borrows its problems' contexts, so a problem box counts the live batches
+ * that hold it.  destroy() of a problem in a live batch throws.  A problem handle
+ * collected while a batch still holds it keeps its context (and box) until the last
+ * such batch is released, whatever order the collector finalizes them in. */
+typedef struct HandleBox {
   void *p;
-  int kind;    /* 1 = problem (FdtdHipContext), 2 = batch (FdtdHipMulti) */
+  int kind;                  /* 1 = problem (FdtdHipContext), 2 = batch (FdtdHipMulti) */
+  int batches;               /* problem: live batches that hold it */
+  int collected;             /* problem: its JS value is gone, release on the last batch */
+  struct HandleBox **members; /* batch: the problem boxes it holds */
+  uint32_t nmembers;
 } HandleBox;
+
+static void problem_release(HandleBox *b) {
+  if (b->p) fdtd_hip_destroy((FdtdHipContext *)b->p);
+  b->p = NULL;
+  free(b);
+}
+
+/* Destroys a batch's library object and lets go of its problems. */
+static void batch_release(HandleBox *b) {
+  if (b->p) fdtd_hip_multi_destroy((FdtdHipMulti *)b->p);
+  b->p = NULL;
+  for (uint32_t q = 0; q < b->nmembers; ++q) {
+    HandleBox *m = b->members[q];
+    if (--m->batches == 0 && m->collected) problem_release(m);
+  }
+  free(b->members);
+  b->members = NULL;
+  b->nmembers = 0;
+}
 
 static void box_finalize(napi_env env, void *data, void *hint) {
   (void)env; (void)hint;
   HandleBox *b = (HandleBox *)data;
-  if (b->p) {
-    if (b->kind == 1) fdtd_hip_destroy((FdtdHipContext *)b->p);
-    else fdtd_hip_multi_destroy((FdtdHipMulti *)b->p);
+  if (b->kind == 1) {
+    if (b->batches > 0) b->collected = 1;
+    else problem_release(b);
+  } else {
+    batch_release(b);
+    free(b);
   }
-  free(b);
 }
 
 static napi_value make_handle(napi_env env, void *p, int kind) {
-  HandleBox *b = (HandleBox *)malloc(sizeof *b);
+  HandleBox *b = (HandleBox *)calloc(1, sizeof *b);
   napi_value ext;
   if (!b) {
     napi_throw_error(env, NULL, "out of host memory");
@@ -536,6 +567,10 @@ static napi_value Destroy(napi_env env, napi_callback_info info) {
   if (!get_args(env, info, 1, argv)) return NULL;
   HandleBox *b = box_of(env, argv[0], 1);
   if (!b) return NULL;
+  if (b->batches > 0) {
+    napi_throw_error(env, NULL, "destroy: the problem is part of a live batch; call multiDestroy on the batch first");
+    return NULL;
+  }
   fdtd_hip_destroy((FdtdHipContext *)b->p);
   b->p = NULL;
   return undefined(env);
@@ -608,11 +643,13 @@ static napi_value MultiCreate(napi_env env, napi_callback_info info) {
     return NULL;
   }
   FdtdHipContext *ctx[4096];
+  HandleBox *box[4096];
   if (n > 4096) { napi_throw_range_error(env, NULL, "at most 4096 problems per batch"); return NULL; }
   for (uint32_t q = 0; q < n; ++q) {
     napi_value e;
     CALL(env, napi_get_element(env, argv[0], q, &e));
-    if (!(ctx[q] = handle(env, e))) return NULL;
+    if (!(box[q] = box_of(env, e, 1))) return NULL;
+    ctx[q] = (FdtdHipContext *)box[q]->p;
   }
   napi_valuetype vt;
   CALL(env, napi_typeof(env, argv[1], &vt));
@@ -622,10 +659,31 @@ static napi_value MultiCreate(napi_env env, napi_callback_info info) {
     if (!tuning_from(env, argv[1], &t)) return NULL;
     tp = &t;
   }
+  HandleBox **members = (HandleBox **)malloc(n * sizeof *members);
+  if (!members) { napi_throw_error(env, NULL, "out of host memory"); return NULL; }
   FdtdHipMulti *m = NULL;
-  if (fdtd_hip_multi_create(ctx, (int)n, tp, &m) != FDTD_HIP_OK) return throw_lib(env, "multiCreate");
+  if (fdtd_hip_multi_create(ctx, (int)n, tp, &m) != FDTD_HIP_OK) {
+    free(members);
+    return throw_lib(env, "multiCreate");
+  }
   napi_value ext = make_handle(env, m, 2);
-  if (!ext) fdtd_hip_multi_destroy(m);
+  if (!ext) {
+    fdtd_hip_multi_destroy(m);
+    free(members);
+    return NULL;
+  }
+  HandleBox *mb = NULL;
+  if (napi_get_value_external(env, ext, (void **)&mb) != napi_ok || !mb) {
+    free(members);     /* the box owns m and releases it when collected */
+    throw_napi(env, "napi_get_value_external");
+    return NULL;
+  }
+  for (uint32_t q = 0; q < n; ++q) {
+    members[q] = box[q];
+    ++box[q]->batches;
+  }
+  mb->members = members;
+  mb->nmembers = n;
   return ext;
 }
 
@@ -664,8 +722,7 @@ static napi_value MultiDestroy(napi_env env, napi_callback_info info) {
   if (!get_args(env, info, 1, argv)) return NULL;
   HandleBox *b = box_of(env, argv[0], 2);
   if (!b) return NULL;
-  fdtd_hip_multi_destroy((FdtdHipMulti *)b->p);
-  b->p = NULL;
+  batch_release(b);
   return undefined(env);
 }
 

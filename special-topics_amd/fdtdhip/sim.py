@@ -114,12 +114,19 @@ class HipSim:
 
     # ------------------------------------------------------------------------------
     def close(self):
+        """Frees the context.  A batch borrows its contexts, so a context in a live
+        HipMulti cannot be closed: close the batch first."""
+        if getattr(self, "_batches", 0):
+            raise RuntimeError("HipSim.close: the context is part of a live HipMulti; close the batch first")
         if getattr(self, "h", None):
             self.L.fdtd_hip_destroy(self.h)
             self.h = None
 
     def __del__(self):
-        self.close()
+        # A live batch holds a reference to this object, so this runs in a batch only at
+        # interpreter exit; the context is then left to the process teardown.
+        if not getattr(self, "_batches", 0):
+            self.close()
 
     def run(self, ts0: int, count: int):
         """Steps [ts0, ts0+count); returns (V[n_sv, count], I[n_si, count])."""
@@ -280,11 +287,15 @@ class HipMulti:
         t = native.FdtdHipTuning(int(planes_per_workgroup), int(cells_per_thread), int(coefficient_storage),
                                  int(fuse_cpml))
         native.check(self.L.fdtd_hip_multi_create(arr, len(self.sims), C.byref(t), C.byref(self.h)), "multi_create")
+        for s in self.sims:
+            s._batches = getattr(s, "_batches", 0) + 1
 
     def close(self):
         if getattr(self, "h", None):
             self.L.fdtd_hip_multi_destroy(self.h)
             self.h = None
+            for s in self.sims:
+                s._batches -= 1
 
     def __del__(self):
         self.close()

@@ -203,9 +203,9 @@ class Sweep:
             # Launch to completion; with several groups it also covers the next group's
             # uploads whenever those take longer than the loop.
             self.loop_seconds += time.perf_counter() - multi.t_launch
+            multi.close()                     # before _finish closes the contexts it borrows
             for j, r in zip(jobs, pool.map(self._finish, jobs)):
                 results[j["idx"]] = r
-            multi.close()
             if self.progress_every:
                 print(f"[sweep] {nxt}/{len(items)} done, {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
             jobs = following

@@ -117,3 +117,20 @@ def test_batch_rejects_misuse(get_problem):
     assert "variants" in L.fdtd_hip_last_error().decode()
     a.close()
     b.close()
+
+
+def test_batch_members_close_after_the_batch(get_problem):
+    """A batch borrows its contexts: closing one while the batch lives is refused and
+    leaves the batch usable; after the batch is closed the contexts close normally."""
+    from fdtdhip.sim import HipMulti
+    prob = get_problem("golden-small")
+    a, b = _sim(prob), _sim(prob)
+    m = HipMulti([a, b])
+    with pytest.raises(RuntimeError, match="live HipMulti"):
+        a.close()
+    v, _ = m.run(0, 3)[0]
+    assert v.shape[1] == 3
+    m.close()
+    a.close()
+    b.close()
+    assert a.h is None and b.h is None
