@@ -90,7 +90,11 @@ inline std::vector<CpmlPiece> cpml_pieces(const CpmlLayout &L, int phase, int a)
 // dealt round-robin to the 8 XCDs, so in a full group one (j, k) tile's consecutive
 // planes -- which read each other's plane as the x neighbour -- run on one XCD.  Timed
 // against plain box order at 256^3 and on the 64-variant IFA batch, the two were
-// equal within run-to-run noise.  The grid axes must not exceed CPML_TILE_MAX_AXIS.
+// equal within run-to-run noise.  Within one (x, y) piece pair, the chunks (one group,
+// one plane) of the z pieces are interleaved: the low-z chunk of a plane is followed by
+// its high-z chunk, and a row's high-z cells and the next row's low-z cells are a few
+// dozen bytes apart in memory, so both tiles of a pair run on one XCD together and
+// share the lines in its L2.  The grid axes must not exceed CPML_TILE_MAX_AXIS.
 inline void build_cpml_tiles(const CpmlLayout &L, int phase, int mode, std::vector<CpmlTile> &out) {
   if (mode == CPML_FUSE_ALL) return;
   std::vector<CpmlPiece> P[3];
@@ -102,7 +106,8 @@ inline void build_cpml_tiles(const CpmlLayout &L, int phase, int mode, std::vect
     return s;
   };
   for (const auto &px : P[0])
-    for (const auto &py : P[1])
+    for (const auto &py : P[1]) {
+      std::vector<std::vector<std::vector<CpmlTile>>> chunks;   // per z piece, per (group, plane)
       for (const auto &pz : P[2]) {
         const unsigned mask = px.mask | py.mask | pz.mask;
         if (!mask) continue;
@@ -112,17 +117,27 @@ inline void build_cpml_tiles(const CpmlLayout &L, int phase, int mode, std::vect
         const int kt = 1 << ks, jt = 1 << js, it = 256 >> (ks + js);
         const int nbk = (pz.hi - pz.lo + kt - 1) / kt, nbj = (py.hi - py.lo + jt - 1) / jt;
         const int nt = nbj * nbk;
+        chunks.emplace_back();
         for (int g0 = 0; g0 < nt; g0 += 8)
-          for (int i0 = px.lo; i0 < px.hi; i0 += it)
+          for (int i0 = px.lo; i0 < px.hi; i0 += it) {
+            std::vector<CpmlTile> chunk;
             for (int t = g0; t < std::min(g0 + 8, nt); ++t) {
               const int j0 = py.lo + (t / nbk) * jt, k0 = pz.lo + (t % nbk) * kt;
               const int j1 = std::min(j0 + jt, py.hi), k1 = std::min(k0 + kt, pz.hi);
               const int i1 = std::min(i0 + it, px.hi);
-              out.push_back(CpmlTile{(unsigned)i0 | mask << 16 | (unsigned)ks << 22 | (unsigned)js << 26,
-                                     (unsigned)j0 | (unsigned)j1 << 16, (unsigned)k0 | (unsigned)k1 << 16,
-                                     (unsigned)i1});
+              chunk.push_back(CpmlTile{(unsigned)i0 | mask << 16 | (unsigned)ks << 22 | (unsigned)js << 26,
+                                       (unsigned)j0 | (unsigned)j1 << 16, (unsigned)k0 | (unsigned)k1 << 16,
+                                       (unsigned)i1});
             }
+            chunks.back().push_back(std::move(chunk));
+          }
       }
+      size_t most = 0;
+      for (const auto &c : chunks) most = std::max(most, c.size());
+      for (size_t q = 0; q < most; ++q)
+        for (const auto &c : chunks)
+          if (q < c.size()) out.insert(out.end(), c[q].begin(), c[q].end());
+    }
 }
 
 // Whether the E CPML pass may change E component `comp` (0..2 = Ex..Ez) at (i, j, k):
