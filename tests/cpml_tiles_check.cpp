@@ -124,8 +124,40 @@ static int check_e_tail() {
   return bad;
 }
 
+// Tile order at 256^3, separate pass: each plane's chunk of low-z tiles (face mask zn
+// only) is followed directly by the same plane's chunk of high-z tiles (zp only), one
+// tile per XCD each, so the two tiles that read a row's last and the next row's first
+// cells run on one XCD together.  Checked over the y-interior rows, whose chunks hold
+// eight tiles (the single-row piece at j = n has one-tile chunks).
+static int check_z_interleave() {
+  const CpmlLayout L = standard(256, 256, 256, 8, 0);
+  int bad = 0;
+  for (int ph = 0; ph < 2; ++ph) {
+    std::vector<CpmlTile> t;
+    build_cpml_tiles(L, ph, CPML_SEPARATE, t);
+    int pairs = 0;
+    for (size_t q = 0; q + 8 < t.size(); ++q) {
+      const unsigned m = (t[q].x >> 16) & 63u;
+      if (m != (1u << 4)) continue;                          // a zn-only tile
+      if ((t[q].y & 0xffffu) < 8 || (t[q].y >> 16) > 248) continue;   // full chunks: y-interior rows
+      const CpmlTile &u = t[q + 8];
+      const bool same_rows = (u.x & 0xffffu) == (t[q].x & 0xffffu) && u.y == t[q].y;
+      if (((u.x >> 16) & 63u) != (1u << 5) || !same_rows) {
+        if (bad++ < 5) std::printf("  phase %d: tile %zu (zn, plane %u) is not followed 8 later by its zp tile\n", ph,
+                                   q, t[q].x & 0xffffu);
+      } else {
+        ++pairs;
+      }
+    }
+    std::printf("256^3 phase %d: %d zn/zp tile pairs 8 apart\n", ph, pairs);
+    if (pairs == 0) ++bad;
+  }
+  return bad;
+}
+
 int main() {
   int bad = check_e_tail();
+  bad += check_z_interleave();
   bad += check(standard(64, 72, 80, 8, 0), "64x72x80 npml 8");
   bad += check(standard(39, 136, 136, 8, 0), "39x136x136 npml 8");
   bad += check(standard(21, 17, 70, 4, 1u << 3), "21x17x70 npml 4 no yp");
