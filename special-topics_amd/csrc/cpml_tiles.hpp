@@ -86,15 +86,17 @@ inline std::vector<CpmlPiece> cpml_pieces(const CpmlLayout &L, int phase, int a)
 // planes the bulk kernels cover (H: all; E: i < nx); none with CPML_FUSE_ALL.  Lanes
 // go along k (the box's k extent rounded up to a power of two, at most 64), then rows
 // along j (its j extent rounded up, up to what is left of 256), then planes along i.
-// Tiles are listed in groups of up to eight (j, k) tiles, plane-minor: workgroups are
-// dealt round-robin to the 8 XCDs, so in a full group one (j, k) tile's consecutive
-// planes -- which read each other's plane as the x neighbour -- run on one XCD.  Timed
-// against plain box order at 256^3 and on the 64-variant IFA batch, the two were
-// equal within run-to-run noise.  Within one (x, y) piece pair, the chunks (one group,
-// one plane) of the z pieces are interleaved: the low-z chunk of a plane is followed by
+// Workgroups are dealt round-robin to the 8 XCDs, and tiles that read the same lines
+// should run on one XCD together.  A piece's tiles are listed in chunks of up to eight
+// row groups (j tiles): for each group of eight, for each plane, for each k tile, one
+// chunk.  In full chunks a row group's k tiles (which share the sectors at their
+// boundaries) and its consecutive planes (which read each other's plane as the x
+// neighbour) are eight positions apart, i.e. on one XCD.  Within one (x, y) piece pair
+// the chunks of the z pieces are interleaved: the low-z chunk of a plane is followed by
 // its high-z chunk, and a row's high-z cells and the next row's low-z cells are a few
-// dozen bytes apart in memory, so both tiles of a pair run on one XCD together and
-// share the lines in its L2.  The grid axes must not exceed CPML_TILE_MAX_AXIS.
+// dozen bytes apart in memory, so both tiles of a pair share the lines in one XCD's L2.
+// At 256^3 the interleave took the H pass from 46.4 to 42.8 us and keeping a row's k
+// tiles on one XCD to 41.8 us.  The grid axes must not exceed CPML_TILE_MAX_AXIS.
 inline void build_cpml_tiles(const CpmlLayout &L, int phase, int mode, std::vector<CpmlTile> &out) {
   if (mode == CPML_FUSE_ALL) return;
   std::vector<CpmlPiece> P[3];
@@ -107,7 +109,7 @@ inline void build_cpml_tiles(const CpmlLayout &L, int phase, int mode, std::vect
   };
   for (const auto &px : P[0])
     for (const auto &py : P[1]) {
-      std::vector<std::vector<std::vector<CpmlTile>>> chunks;   // per z piece, per (group, plane)
+      std::vector<std::vector<std::vector<CpmlTile>>> chunks;   // per z piece, per (group, plane, k tile)
       for (const auto &pz : P[2]) {
         const unsigned mask = px.mask | py.mask | pz.mask;
         if (!mask) continue;
@@ -116,21 +118,21 @@ inline void build_cpml_tiles(const CpmlLayout &L, int phase, int mode, std::vect
         const int js = log2_up(py.hi - py.lo, 8 - ks);
         const int kt = 1 << ks, jt = 1 << js, it = 256 >> (ks + js);
         const int nbk = (pz.hi - pz.lo + kt - 1) / kt, nbj = (py.hi - py.lo + jt - 1) / jt;
-        const int nt = nbj * nbk;
         chunks.emplace_back();
-        for (int g0 = 0; g0 < nt; g0 += 8)
-          for (int i0 = px.lo; i0 < px.hi; i0 += it) {
-            std::vector<CpmlTile> chunk;
-            for (int t = g0; t < std::min(g0 + 8, nt); ++t) {
-              const int j0 = py.lo + (t / nbk) * jt, k0 = pz.lo + (t % nbk) * kt;
-              const int j1 = std::min(j0 + jt, py.hi), k1 = std::min(k0 + kt, pz.hi);
-              const int i1 = std::min(i0 + it, px.hi);
-              chunk.push_back(CpmlTile{(unsigned)i0 | mask << 16 | (unsigned)ks << 22 | (unsigned)js << 26,
-                                       (unsigned)j0 | (unsigned)j1 << 16, (unsigned)k0 | (unsigned)k1 << 16,
-                                       (unsigned)i1});
+        for (int g0 = 0; g0 < nbj; g0 += 8)
+          for (int i0 = px.lo; i0 < px.hi; i0 += it)
+            for (int kb = 0; kb < nbk; ++kb) {
+              std::vector<CpmlTile> chunk;
+              for (int jb = g0; jb < std::min(g0 + 8, nbj); ++jb) {
+                const int j0 = py.lo + jb * jt, k0 = pz.lo + kb * kt;
+                const int j1 = std::min(j0 + jt, py.hi), k1 = std::min(k0 + kt, pz.hi);
+                const int i1 = std::min(i0 + it, px.hi);
+                chunk.push_back(CpmlTile{(unsigned)i0 | mask << 16 | (unsigned)ks << 22 | (unsigned)js << 26,
+                                         (unsigned)j0 | (unsigned)j1 << 16, (unsigned)k0 | (unsigned)k1 << 16,
+                                         (unsigned)i1});
+              }
+              chunks.back().push_back(std::move(chunk));
             }
-            chunks.back().push_back(std::move(chunk));
-          }
       }
       size_t most = 0;
       for (const auto &c : chunks) most = std::max(most, c.size());
