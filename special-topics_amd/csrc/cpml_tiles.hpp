@@ -91,12 +91,15 @@ inline std::vector<CpmlPiece> cpml_pieces(const CpmlLayout &L, int phase, int a)
 // row groups (j tiles): for each group of eight, for each plane, for each k tile, one
 // chunk.  In full chunks a row group's k tiles (which share the sectors at their
 // boundaries) and its consecutive planes (which read each other's plane as the x
-// neighbour) are eight positions apart, i.e. on one XCD.  Within one (x, y) piece pair
+// neighbour) are eight positions apart, i.e. on one XCD.  A piece with fewer than
+// eight row groups fills its chunks with the row groups of several planes, which keeps
+// a row's k tiles on one XCD; its planes' x neighbours then sit on other XCDs, which
+// matters only for x-face tiles, and those come from pieces with many row groups.  Within one (x, y) piece pair
 // the chunks of the z pieces are interleaved: the low-z chunk of a plane is followed by
 // its high-z chunk, and a row's high-z cells and the next row's low-z cells are a few
 // dozen bytes apart in memory, so both tiles of a pair share the lines in one XCD's L2.
 // At 256^3 the interleave took the H pass from 46.4 to 42.8 us and keeping a row's k
-// tiles on one XCD to 41.8 us.  The grid axes must not exceed CPML_TILE_MAX_AXIS.
+// tiles on one XCD to 39.7 us.  The grid axes must not exceed CPML_TILE_MAX_AXIS.
 inline void build_cpml_tiles(const CpmlLayout &L, int phase, int mode, std::vector<CpmlTile> &out) {
   if (mode == CPML_FUSE_ALL) return;
   std::vector<CpmlPiece> P[3];
@@ -118,19 +121,22 @@ inline void build_cpml_tiles(const CpmlLayout &L, int phase, int mode, std::vect
         const int js = log2_up(py.hi - py.lo, 8 - ks);
         const int kt = 1 << ks, jt = 1 << js, it = 256 >> (ks + js);
         const int nbk = (pz.hi - pz.lo + kt - 1) / kt, nbj = (py.hi - py.lo + jt - 1) / jt;
+        // Planes per chunk: one with eight or more row groups, else enough to fill it.
+        const int per = nbj >= 8 ? 1 : 8 / nbj;
         chunks.emplace_back();
         for (int g0 = 0; g0 < nbj; g0 += 8)
-          for (int i0 = px.lo; i0 < px.hi; i0 += it)
+          for (int i0 = px.lo; i0 < px.hi; i0 += it * per)
             for (int kb = 0; kb < nbk; ++kb) {
               std::vector<CpmlTile> chunk;
-              for (int jb = g0; jb < std::min(g0 + 8, nbj); ++jb) {
-                const int j0 = py.lo + jb * jt, k0 = pz.lo + kb * kt;
-                const int j1 = std::min(j0 + jt, py.hi), k1 = std::min(k0 + kt, pz.hi);
-                const int i1 = std::min(i0 + it, px.hi);
-                chunk.push_back(CpmlTile{(unsigned)i0 | mask << 16 | (unsigned)ks << 22 | (unsigned)js << 26,
-                                         (unsigned)j0 | (unsigned)j1 << 16, (unsigned)k0 | (unsigned)k1 << 16,
-                                         (unsigned)i1});
-              }
+              for (int ib = i0; ib < std::min(i0 + it * per, px.hi); ib += it)
+                for (int jb = g0; jb < std::min(g0 + 8, nbj); ++jb) {
+                  const int j0 = py.lo + jb * jt, k0 = pz.lo + kb * kt;
+                  const int j1 = std::min(j0 + jt, py.hi), k1 = std::min(k0 + kt, pz.hi);
+                  const int i1 = std::min(ib + it, px.hi);
+                  chunk.push_back(CpmlTile{(unsigned)ib | mask << 16 | (unsigned)ks << 22 | (unsigned)js << 26,
+                                           (unsigned)j0 | (unsigned)j1 << 16, (unsigned)k0 | (unsigned)k1 << 16,
+                                           (unsigned)i1});
+                }
               chunks.back().push_back(std::move(chunk));
             }
       }
