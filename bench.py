@@ -28,8 +28,21 @@ REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(REPO, "special-topics_amd"))
 
 HBM_PEAK_GBS = 8000.0          # MI355X HBM3E
-FP32_BYTES_PER_CELL_PHASE = 72  # 3 own-field reads + 3 writes + 3 neighbour-field reads + 9 coefficients, x4 B
-PSI_BYTES = 12                  # read Psi, write Psi, read CPsi (FP32)
+FIELD_VALUES_PER_CELL_PHASE = 9  # 3 own-field reads + 3 writes + 3 neighbour-field reads
+REF_VALUES_PER_CELL_PHASE = 18   # the same plus the reference's 9 per-cell coefficients
+PSI_VALUES = 3                   # read Psi, write Psi, read CPsi
+
+
+def cell_phase_bytes(esz: int, coef_mode: int) -> int:
+    """Compulsory HBM bytes of one cell in one stencil phase for the coefficient storage
+    in use: 9 field values, plus 3 per-cell coefficient indices (8- or 16-bit; the
+    deduplicated coefficient tables themselves stay on chip) or 9 per-cell
+    coefficient values."""
+    if coef_mode == 1:
+        return FIELD_VALUES_PER_CELL_PHASE * esz + 3
+    if coef_mode == 2:
+        return FIELD_VALUES_PER_CELL_PHASE * esz + 6
+    return REF_VALUES_PER_CELL_PHASE * esz
 
 
 def psi_counts(problem, axes="xyz"):
@@ -73,16 +86,15 @@ def cpu_baseline(problem64, steps: int, threads: int, target_s: float = 15.0):
     return nx * ny * nz * steps / dt / 1e6, dt, steps
 
 
-def measured_traffic(workload: str, precision: str, kernel: str):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE /
-    WRITE_SIZE summary of this workload (calibrated against a 1 GiB copy), or None."""
+def traffic_table(workload: str, precision: str) -> dict:
+    """Per-kernel HBM bytes per launch from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary of this workload (calibrated against a 1 GiB copy); {} if none."""
     path = os.path.join(REPO, "profiles", f"{workload}-{precision}-traffic.json")
     try:
         with open(path) as f:
-            k = json.load(f)["kernels"].get(kernel)
+            return {k: int(v["bytes"]) for k, v in json.load(f)["kernels"].items()}
     except (OSError, ValueError, KeyError):
-        return None
-    return int(k["bytes"]) if k else None
+        return {}
 
 
 def pmc_summary(fetch_dir: str, write_dir: str, copy_mib: int, out: str):
@@ -263,6 +275,124 @@ def early_stop_study(count=4, batch_count=64, first=4600):
     return out
 
 
+def stencil_study(workload, precision, steps, warmup, timing_steps, workdir, ntff_mode="auto", fixed=None):
+    """Whole-step throughput of `workload` and the HBM-roofline position of its
+    dominant stencil kernel.
+
+    Throughput: W untimed steps, then exactly K steps between device-wide syncs.
+    Kernel durations: HIP events recorded on the library's stream around each launch
+    kind, over `timing_steps` steps of a fresh run with the same tuning.  `achieved` is
+    the kernel's algorithmic bytes for the coefficient storage in use (cell_phase_bytes
+    per cell, plus the Psi bytes of CPML faces fused into it) over its average
+    duration; `traffic` is what rocprofv3's calibrated FETCH_SIZE + WRITE_SIZE counted
+    for the same kernel (profiles/<workload>-<precision>-traffic.json), and
+    `traffic_frac` the fraction of peak those counted bytes give."""
+    import torch
+    from fdtdhip.problem import load_scenario_problem
+    from fdtdhip.sim import HipSim
+
+    fp32 = precision == "fp32"
+    t_setup = time.perf_counter()
+    progress(f"building {workload} ({precision})")
+    problem = load_scenario_problem(workload, workdir, steps=max(warmup + steps, timing_steps + 1),
+                                    fp32=fp32, no_phase=False)
+    nx, ny, nz = problem.dims
+    ncells = nx * ny * nz
+    progress(f"uploading {nx}x{ny}x{nz}")
+    sim = HipSim(problem, precision=precision)
+    sim.set_ntff_mode(ntff_mode)
+    progress("tuning")
+    if fixed is None:
+        tuning = sim.autotune(3)
+    else:
+        sim.set_tuning(*fixed)
+        tuning = sim.tuning()
+    setup_s = time.perf_counter() - t_setup
+
+    progress(f"timing {steps} steps with {tuning}")
+    sim.run_no_readback(0, warmup)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    sim.run_no_readback(warmup, steps)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    sim.close()
+
+    sim = HipSim(problem, precision=precision)
+    sim.set_ntff_mode(ntff_mode)
+    sim.set_tuning(*tuning)
+    sim.set_timing(True)
+    sim.run_no_readback(0, timing_steps)
+    ms, launches = sim.kernel_times()
+    coef_mode = sim.coefficient_storage()[0]
+    ntff_blocked = sim.ntff_blocked() if sim.nfreq else None
+    sim.close()
+    avg_ms = [m / max(n, 1) for m, n in zip(ms, launches)]
+
+    esz = 4 if fp32 else 8
+    nh, ne = psi_counts(problem)
+    cb = cell_phase_bytes(esz, coef_mode)
+    dom = 0 if avg_ms[0] >= avg_ms[1] else 1
+    fused_axes = {0: "", 1: "z", 2: "xyz"}[tuning[3]]
+    fh, fe = psi_counts(problem, fused_axes) if fused_axes else (0, 0)
+    psi_fused = PSI_VALUES * esz * (fh if dom == 0 else fe)
+    dom_bytes = cb * ncells + psi_fused
+    ref_bytes = REF_VALUES_PER_CELL_PHASE * esz * ncells + psi_fused
+    dom_s = avg_ms[dom] * 1e-3
+    achieved = dom_bytes / dom_s / 1e9
+
+    ctype = "float" if fp32 else "double"
+    kname = f"fdtd::k_update_{'h' if dom == 0 else 'e'}<{ctype}, {coef_mode}, {max(tuning[1], 1)}, {tuning[3]}>"
+    table = traffic_table(workload, precision)
+    traffic = table.get(kname)
+    step_s = elapsed / steps
+    # Whole step, counted: the two stencils, the two CPML passes and the small E-tail
+    # kernels as the timing steps launch them (separately).
+    step_kernels = [f"fdtd::k_update_h<{ctype}, {coef_mode}, {max(tuning[1], 1)}, {tuning[3]}>",
+                    f"fdtd::k_update_e<{ctype}, {coef_mode}, {max(tuning[1], 1)}, {tuning[3]}>"]
+    if tuning[3] != 2:
+        step_kernels += [f"fdtd::k_cpml_h<{ctype}>", f"fdtd::k_cpml_e<{ctype}>"]
+    step_traffic = sum(table[k] for k in step_kernels) if all(k in table for k in step_kernels) else None
+    step_bytes = 2 * cb * ncells + PSI_VALUES * esz * (nh + ne)
+
+    return {
+        "workload": f"{workload} {nx}x{ny}x{nz} {precision.upper()} fields+coefficients, FP64 port DFT/traces",
+        "cells": ncells,
+        "mcell_updates_per_s": round(ncells * steps / elapsed / 1e6, 1),
+        "ms_per_step": round(step_s * 1e3, 4),
+        "roofline": {
+            "bound": "hbm",
+            "kernel": kname,
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": traffic,
+            "traffic_frac": round(traffic / dom_s / 1e9 / HBM_PEAK_GBS, 4) if traffic else None,
+            "algorithmic_bytes_per_launch": dom_bytes,
+            "bytes_per_cell": cb,
+            "avg_launch_ms": round(avg_ms[dom], 5),
+            # The same kernel priced as if it read the reference's 9 per-cell coefficient
+            # arrays (18 values per cell): the rate the indexed storage is worth.
+            "algorithmic_equivalent_frac": round(ref_bytes / dom_s / 1e9 / HBM_PEAK_GBS, 4),
+        },
+        "step_roofline": {
+            "algorithmic_bytes": step_bytes,
+            "frac": round(step_bytes / step_s / 1e9 / HBM_PEAK_GBS, 4),
+            "traffic": step_traffic,
+            "traffic_frac": round(step_traffic / step_s / 1e9 / HBM_PEAK_GBS, 4) if step_traffic else None,
+        },
+        "kernels_avg_ms": {"update_h": round(avg_ms[0], 5), "cpml_h": round(avg_ms[4], 5),
+                           "update_e": round(avg_ms[1], 5), "cpml_e": round(avg_ms[5], 5),
+                           "port_epilogue": round(avg_ms[2], 5), "ntff": round(avg_ms[3], 5)},
+        "ntff_mode": None if ntff_blocked is None else ("blocked (matrix cores)" if ntff_blocked else "direct"),
+        "setup_s": round(setup_s, 2),
+        "tuning": {"planes_per_workgroup": tuning[0], "cells_per_thread": tuning[1],
+                   "coefficient_storage": ["indexed", "arrays"][tuning[2]],
+                   "cpml": ["separate pass", "z slabs fused into bulk", "all faces fused into bulk"][tuning[3]]},
+    }
+
+
 def progress(msg: str):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -289,6 +419,8 @@ def main():
     ap.add_argument("--sweep-batch", type=int, default=64,
                     help="variants per batched launch in the sweep study (later groups upload while the previous one runs)")
     ap.add_argument("--no-early-stop", action="store_true", help="skip the early-stop time-to-solution study")
+    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 roofline-256 sub-measurement")
+    ap.add_argument("--no-patch512", action="store_true", help="skip the 512x512x384 sub-measurement")
     ap.add_argument("--ntff-mode", default="auto", choices=["auto", "direct", "blocked"],
                     help="NTFF running DFT: step-by-step (direct) or K-blocked on the matrix cores")
     ap.add_argument("--workdir", default=os.path.join(tempfile.gettempdir(), "fdtd-bench"))
@@ -314,8 +446,6 @@ def main():
         sys.exit(2)
 
     import torch
-    from fdtdhip.problem import load_scenario_problem
-    from fdtdhip.sim import HipSim
 
     os.makedirs(args.workdir, exist_ok=True)
     if args.copy_calibration_mib:
@@ -326,103 +456,35 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
+    fixed = None if args.planes < 0 else (args.planes, args.cells, args.coef, args.fuse_cpml)
+    st = stencil_study(args.workload, args.precision, args.steps, args.warmup, args.timing_steps, args.workdir,
+                       args.ntff_mode, fixed)
     fp32 = args.precision == "fp32"
-    total_steps = args.warmup + args.steps
-    t_setup = time.perf_counter()
-    progress(f"building {args.workload}")
-    problem = load_scenario_problem(args.workload, args.workdir, steps=max(total_steps, args.timing_steps + 1),
-                                    fp32=fp32, no_phase=False)
-    nx, ny, nz = problem.dims
-    ncells = nx * ny * nz
-    progress(f"uploading {nx}x{ny}x{nz}")
-    sim = HipSim(problem, precision=args.precision)
-    sim.set_ntff_mode(args.ntff_mode)
-    progress("tuning")
-    if args.planes < 0:
-        tuning = sim.autotune(3)
-    else:
-        sim.set_tuning(args.planes, args.cells, args.coef, args.fuse_cpml)
-        tuning = sim.tuning()
-    setup_s = time.perf_counter() - t_setup
-
-    # Throughput: W untimed steps, then exactly K steps between device-wide syncs.
-    progress(f"timing {args.steps} steps with {tuning}")
-    sim.run_no_readback(0, args.warmup)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    sim.run_no_readback(args.warmup, args.steps)
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-    sim.close()
-
-    # Per-kernel durations with HIP events on the library stream (fresh state).
-    sim = HipSim(problem, precision=args.precision)
-    sim.set_ntff_mode(args.ntff_mode)
-    sim.set_tuning(*tuning)
-    sim.set_timing(True)
-    sim.run_no_readback(0, args.timing_steps)
-    ms, launches = sim.kernel_times()
-    coef_mode = sim.coefficient_storage()[0]
-    ntff_blocked = sim.ntff_blocked() if sim.nfreq else None
-    sim.close()
-    avg_ms = [m / max(n, 1) for m, n in zip(ms, launches)]
-    nh, ne = psi_counts(problem)
-    esz = 4 if fp32 else 8
-    # Algorithmic bytes: the bulk kernels move 18 values per cell (3 own-field reads and
-    # writes, 3 neighbour-field reads, 9 coefficients); the CPML kernels 3 values per
-    # Psi element (Psi read and write, CPsi read).
-    bulk_bytes = FP32_BYTES_PER_CELL_PHASE * ncells * esz // 4
-    bytes_h = bulk_bytes + PSI_BYTES * nh * esz // 4
-    bytes_e = bulk_bytes + PSI_BYTES * ne * esz // 4
-    dom = 0 if avg_ms[0] >= avg_ms[1] else 1
-    # A stencil kernel that applies CPML terms itself also moves those faces' Psi bytes.
-    fused_axes = {0: "", 1: "z", 2: "xyz"}[tuning[3]]
-    fh, fe = psi_counts(problem, fused_axes) if fused_axes else (0, 0)
-    dom_bytes = bulk_bytes + PSI_BYTES * (fh if dom == 0 else fe) * esz // 4
-    achieved = dom_bytes / (avg_ms[dom] * 1e-3) / 1e9
-
-    ctype = "float" if fp32 else "double"
-    kname = f"fdtd::k_update_{'h' if dom == 0 else 'e'}<{ctype}, {coef_mode}, {max(tuning[1], 1)}, {tuning[3]}>"
-    traffic = measured_traffic(args.workload, args.precision, kname)
-
-    mcells = ncells * args.steps / elapsed / 1e6
     line = {
         "metric": "Mcell-updates/s",
-        "value": round(mcells, 1),
+        "value": st["mcell_updates_per_s"],
         "unit": "Mcell-updates/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(elapsed / args.steps * 1e3, 4),
+        "ms_per_step": st["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32" if fp32 else "f64",
         "data": "synthetic",
-        "config": {"workload": f"{args.workload} {nx}x{ny}x{nz} {args.precision.upper()} fields+coefficients, "
-                               "FP64 port DFT/traces", "cells": ncells, "precision": args.precision,
+        "config": {"workload": st["workload"], "cells": st["cells"], "precision": args.precision,
                    "parallelism": "single GPU"},
-        "roofline": {
-            "bound": "hbm",
-            "kernel": kname,
-            "achieved": round(achieved, 1),
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": traffic,
-            "algorithmic_bytes_per_launch": dom_bytes,
-            "avg_launch_ms": round(avg_ms[dom], 5),
-        },
-        "kernels_avg_ms": {"update_h": round(avg_ms[0], 5), "cpml_h": round(avg_ms[4], 5),
-                           "update_e": round(avg_ms[1], 5), "cpml_e": round(avg_ms[5], 5),
-                           "port_epilogue": round(avg_ms[2], 5), "ntff": round(avg_ms[3], 5)},
-        "ntff_mode": None if ntff_blocked is None else ("blocked (matrix cores)" if ntff_blocked else "direct"),
-        "hbm_equivalent_frac_step": round((bytes_h + bytes_e) / (elapsed / args.steps) / 1e9 / HBM_PEAK_GBS, 4),
-        "setup_s": round(setup_s, 2),
-        "tuning": {"planes_per_workgroup": tuning[0], "cells_per_thread": tuning[1],
-                   "coefficient_storage": ["indexed", "arrays"][tuning[2]],
-                   "cpml": ["separate pass", "z slabs fused into bulk", "all faces fused into bulk"][tuning[3]]},
     }
+    line.update({k: v for k, v in st.items() if k not in ("mcell_updates_per_s", "ms_per_step", "workload", "cells")})
+
+    if args.workload == "roofline-256" and not args.no_fp64:
+        progress("FP64 roofline-256 (the reference's precision)")
+        line["fp64_roofline_256"] = stencil_study("roofline-256", "fp64", 500, 10, 60, args.workdir, args.ntff_mode)
+
+    if args.workload == "roofline-256" and not args.no_patch512:
+        progress("patch-512 (512x512x384 FP32, NTFF at 16 f)")
+        line["patch_512"] = stencil_study("patch-512", "fp32", 200, 10, 70, args.workdir, args.ntff_mode)
 
     if not args.no_ifa:
         progress("IFA FP32-vs-FP64 study")
@@ -437,8 +499,10 @@ def main():
         line["early_stop"] = early_stop_study()
 
     if not args.no_cpu_baseline:
+        from fdtdhip.problem import load_scenario_problem
         progress("CPU baseline")
         threads = max(1, min(host_cores() - 1, 15))   # the box gives one GPU a 16-core share
+        total_steps = args.warmup + args.steps
         p64 = load_scenario_problem(args.workload, args.workdir, steps=max(total_steps, args.timing_steps + 1),
                                     fp32=False, no_phase=False)
         mc, secs, steps_cpu = cpu_baseline(p64, args.cpu_steps, threads)
@@ -446,6 +510,15 @@ def main():
             "value": round(mc, 2), "unit": "Mcell-updates/s", "cores": threads, "kind": "port",
             "sample": f"{steps_cpu} FP64 steps of {args.workload} ({secs:.1f} s): x-slab threads for the bulk "
                       "curls, CPML/captures on one thread, as the reference's wasm-cpu engine is structured",
+        }
+        # The reference's own published case: the IFA, 39x136x136, FP64 (published:
+        # ~22 ms/step = 32.8 Mcell/s on 16 worker threads).
+        pifa = load_scenario_problem("ifa-matlab-7000", args.workdir)
+        mc, secs, steps_cpu = cpu_baseline(pifa, 0, threads, target_s=10.0)
+        line["cpu_baseline"]["ifa_fp64"] = {
+            "value": round(mc, 2), "unit": "Mcell-updates/s", "cores": threads,
+            "ms_per_step": round(secs / steps_cpu * 1e3, 2),
+            "sample": f"{steps_cpu} FP64 steps of ifa-matlab-7000 39x136x136 ({secs:.1f} s)",
         }
     print(json.dumps(line))
 

@@ -132,9 +132,21 @@ int fdtd_hip_upload_farfield(FdtdHipContext *ctx, int nFreq, const int *box,
 
 /* Running FP64 DFTs of every sampled voltage and current at nFreq frequencies:
  * X(f) = dt * sum_n x_n exp(-j 2 pi f ((n+1) dt + shift)), shift 0 for voltages and
- * -dt/2 for currents (dft.js:17-38 as called at index.js:69-70).  No counterpart in
- * the reference ABI (the reference transforms the traces after the loop). */
+ * -dt/2 for currents (dft.js:17-38 as called at index.js:69-70).  May come before or
+ * after the observer uploads: the accumulators are sized at the first step from the
+ * observers uploaded by then (observers are refused after the first step).  No
+ * counterpart in the reference ABI (the reference transforms the traces after the
+ * loop). */
 int fdtd_hip_upload_port_dft(FdtdHipContext *ctx, int nFreq, const double *freqHz);
+
+/* Object counts of a context, so that a binding can size and check the arrays it
+ * passes to the calls below: sampled voltages and currents, point field probes,
+ * voltage sources, NTFF frequencies and port-DFT frequencies (0 when not uploaded).
+ * No counterpart in the reference ABI (its binding trusts the caller's lengths). */
+typedef struct {
+    int nSampledVoltages, nSampledCurrents, nProbes, nSources, nFarfieldFreq, nPortDftFreq;
+} FdtdHipCounts;
+int fdtd_hip_get_counts(FdtdHipContext *ctx, FdtdHipCounts *out);
 
 /* Run steps [tsStart, tsStart+count).  svOut: [nSampledVoltages][count],
  * siOut: [nSampledCurrents][count], FP64, either may be NULL.

@@ -299,6 +299,11 @@ int ensure_traces(FdtdHipContext *c) {
   c->ports.pr = prd;
   c->ports.steps = steps;
   c->ports.dt = c->d.dt;
+  // The running port DFT covers the observers uploaded by now; no more can follow.
+  if (c->ndft && !c->dft_acc) {
+    const int ntr = std::max(1, c->ports.nsv + c->ports.nsi);
+    if (upload_small(c, std::vector<double>((size_t)2 * ntr * c->ndft, 0.0), &c->dft_acc)) return FDTD_HIP_ERR;
+  }
   c->traces_ready = 1;
   return FDTD_HIP_OK;
 }
@@ -957,9 +962,23 @@ int fdtd_hip_upload_port_dft(FdtdHipContext *c, int nFreq, const double *freqHz)
   std::vector<double> w(nFreq);
   for (int f = 0; f < nFreq; ++f) w[f] = 2 * M_PI * freqHz[f];
   if (upload_small(c, w, &c->dft_w)) return FDTD_HIP_ERR;
-  const int ntr = std::max(1, c->ports.nsv + c->ports.nsi);
-  if (upload_small(c, std::vector<double>((size_t)2 * ntr * nFreq, 0.0), &c->dft_acc)) return FDTD_HIP_ERR;
   c->ndft = nFreq;
+  // Accumulators for the observers present at the first step (ensure_traces).
+  if (c->traces_ready) {
+    const int ntr = std::max(1, c->ports.nsv + c->ports.nsi);
+    if (upload_small(c, std::vector<double>((size_t)2 * ntr * nFreq, 0.0), &c->dft_acc)) return FDTD_HIP_ERR;
+  }
+  return FDTD_HIP_OK;
+}
+
+int fdtd_hip_get_counts(FdtdHipContext *c, FdtdHipCounts *out) {
+  if (!c || !out) return fail("get_counts: null argument");
+  out->nSampledVoltages = c->ports.nsv;
+  out->nSampledCurrents = c->ports.nsi;
+  out->nProbes = c->ports.npr;
+  out->nSources = c->ports.nvs;
+  out->nFarfieldFreq = c->nff;
+  out->nPortDftFreq = c->ndft;
   return FDTD_HIP_OK;
 }
 
@@ -1140,6 +1159,11 @@ int fdtd_hip_read_port_dft(FdtdHipContext *c, double *re, double *im, long long 
   if (!c->ndft) return fail("read_port_dft: no port DFT uploaded");
   const long long want = (long long)(c->ports.nsv + c->ports.nsi) * c->ndft;
   if (n != want) return fail("read_port_dft: %lld elements, expected %lld", n, want);
+  if (!c->dft_acc) {                     // no step has run: every transform is zero
+    std::fill(re, re + n, 0.0);
+    std::fill(im, im + n, 0.0);
+    return FDTD_HIP_OK;
+  }
   std::vector<double> v(2 * n);
   HIP_TRY(hipMemcpyAsync(v.data(), c->dft_acc, 2 * n * 8, hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
@@ -1192,36 +1216,59 @@ int fdtd_hip_radiated_power(FdtdHipContext *c, double *out, int nFreq) {
   return FDTD_HIP_OK;
 }
 
-int fdtd_hip_directivity(FdtdHipContext *c, const double *freqHz, const double *radPower, int nAngles,
-                         const double *theta, const double *phi, double *dataTheta, double *dataPhi) {
-  if (!c || !freqHz || !theta || !phi || !dataTheta || !dataPhi) return fail("directivity: null argument");
-  if (!c->nff) return fail("directivity: no far-field box uploaded");
-  if (nAngles <= 0) return fail("directivity: %d angles", nAngles);
+// Directivity of a context into device workspace `ws` laid out by DirLayout for
+// capacities (nf_cap, na_cap): inputs staged from the host, the trigonometry of the
+// angles and the wave numbers on the device, the radiated power (unless given) and the
+// directivity kernel, outputs copied back.
+struct DirLayout {
+  long kw, prad, freq, theta, phi, ang, dth, dph, total;
+  DirLayout(long nf_cap, long na_cap, long out_cap) {
+    kw = 0; prad = nf_cap; freq = 2 * nf_cap; theta = 3 * nf_cap; phi = theta + na_cap;
+    ang = phi + na_cap; dth = ang + 4 * na_cap; dph = dth + out_cap; total = dph + out_cap;
+  }
+};
+
+static int directivity_impl(FdtdHipContext *c, const double *freqHz, const double *radPower, int nAngles,
+                            const double *theta, const double *phi, double *dataTheta, double *dataPhi,
+                            double *ws, const DirLayout &L) {
   const int nf = c->nff;
-  std::vector<double> prad(nf), kw(nf), ang(4 * (size_t)nAngles);
-  if (radPower) {
-    std::copy(radPower, radPower + nf, prad.begin());
-  } else if (fdtd_hip_radiated_power(c, prad.data(), nf)) {
-    return FDTD_HIP_ERR;
-  }
-  for (int f = 0; f < nf; ++f) kw[f] = 2 * M_PI * freqHz[f] / 2.99792458e8;
-  for (int a = 0; a < nAngles; ++a) {
-    ang[4 * a] = std::sin(theta[a]); ang[4 * a + 1] = std::cos(theta[a]);
-    ang[4 * a + 2] = std::sin(phi[a]); ang[4 * a + 3] = std::cos(phi[a]);
-  }
-  Scratch s;
-  const double *dk = s.put(c, kw.data(), nf), *dp = s.put(c, prad.data(), nf);
-  const double *da = s.put(c, ang.data(), ang.size());
-  double *dt = s.put<double>(c, nullptr, (size_t)nf * nAngles), *dq = s.put<double>(c, nullptr, (size_t)nf * nAngles);
-  if (!dk || !dp || !da || !dt || !dq) return fail("directivity: device allocation failed");
+  const size_t nout = (size_t)nf * nAngles;
+  HIP_TRY(hipMemcpyAsync(ws + L.freq, freqHz, nf * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(hipMemcpyAsync(ws + L.theta, theta, nAngles * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(hipMemcpyAsync(ws + L.phi, phi, nAngles * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  if (radPower)
+    HIP_TRY(hipMemcpyAsync(ws + L.prad, radPower, nf * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  else
+    hipLaunchKernelGGL(k_ntff_power, dim3(nf), dim3(256), 0, c->stream, ntff_post_params(c), ws + L.prad);
+  const int nprep = std::max(nf, nAngles);
+  hipLaunchKernelGGL(k_directivity_prep, dim3((nprep + 255) / 256), dim3(256), 0, c->stream, ws + L.freq,
+                     ws + L.theta, ws + L.phi, nf, nAngles, ws + L.kw, ws + L.ang);
   constexpr int NA = 4;
   hipLaunchKernelGGL(k_ntff_directivity<NA>, dim3((nAngles + NA - 1) / NA, nf), dim3(256), 0, c->stream,
-                     ntff_post_params(c), dk, dp, da, nAngles, dt, dq);
+                     ntff_post_params(c), ws + L.kw, ws + L.prad, ws + L.ang, nAngles, ws + L.dth, ws + L.dph);
   HIP_TRY(hipGetLastError());
-  HIP_TRY(hipMemcpyAsync(dataTheta, dt, (size_t)nf * nAngles * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-  HIP_TRY(hipMemcpyAsync(dataPhi, dq, (size_t)nf * nAngles * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipMemcpyAsync(dataTheta, ws + L.dth, nout * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipMemcpyAsync(dataPhi, ws + L.dph, nout * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
   return FDTD_HIP_OK;
+}
+
+static int directivity_check(FdtdHipContext *c, const double *freqHz, const double *theta, const double *phi,
+                             const double *dataTheta, const double *dataPhi, int nAngles, const char *who) {
+  if (!c || !freqHz || !theta || !phi || !dataTheta || !dataPhi) return fail("%s: null argument", who);
+  if (!c->nff) return fail("%s: no far-field box uploaded", who);
+  if (nAngles <= 0) return fail("%s: %d angles", who, nAngles);
+  return FDTD_HIP_OK;
+}
+
+int fdtd_hip_directivity(FdtdHipContext *c, const double *freqHz, const double *radPower, int nAngles,
+                         const double *theta, const double *phi, double *dataTheta, double *dataPhi) {
+  if (directivity_check(c, freqHz, theta, phi, dataTheta, dataPhi, nAngles, "directivity")) return FDTD_HIP_ERR;
+  const DirLayout L(c->nff, nAngles, (long)c->nff * nAngles);
+  Scratch s;
+  double *ws = s.put<double>(c, nullptr, (size_t)L.total);
+  if (!ws) return fail("directivity: device allocation failed");
+  return directivity_impl(c, freqHz, radPower, nAngles, theta, phi, dataTheta, dataPhi, ws, L);
 }
 
 int fdtd_hip_set_ntff_mode(FdtdHipContext *c, int mode) {

@@ -104,6 +104,21 @@ static int get_args(napi_env env, napi_callback_info info, size_t want, napi_val
   return 1;
 }
 
+/* The context's object counts (fdtd_hip_get_counts), with a thrown error on failure. */
+static int counts_of(napi_env env, FdtdHipContext *c, FdtdHipCounts *out, const char *who) {
+  if (fdtd_hip_get_counts(c, out) != FDTD_HIP_OK) { throw_lib(env, who); return 0; }
+  return 1;
+}
+
+/* RangeError unless a typed array of n elements holds per * count values. */
+static int need(napi_env env, size_t n, long long per, int count, const char *what) {
+  if (count < 0 || (long long)n >= per * (long long)count) return 1;
+  char buf[256];
+  snprintf(buf, sizeof buf, "%s: %zu elements, need %lld (%lld x %d steps)", what, n, per * (long long)count, per, count);
+  napi_throw_range_error(env, NULL, buf);
+  return 0;
+}
+
 static int get_int(napi_env env, napi_value v, int *out) {
   if (napi_get_value_int32(env, v, out) != napi_ok) { napi_throw_type_error(env, NULL, "expected a number"); return 0; }
   return 1;
@@ -264,9 +279,16 @@ static napi_value UploadCpml(napi_env env, napi_callback_info info) {
   const char *pk[4] = {"b_e", "a_e", "b_m", "a_m"};
   double *prof[4];
   for (int q = 0; q < 4; ++q) {
+    size_t n;
     if (!prop(env, o, pk[q], &v)) return NULL;
-    prof[q] = typed(env, v, napi_float64_array, NULL, 0, &ok);
+    prof[q] = typed(env, v, napi_float64_array, &n, 0, &ok);
     if (!ok) return NULL;
+    if (nc < 0 || n < (size_t)nc) {
+      char buf[160];
+      snprintf(buf, sizeof buf, "uploadCpml: profile %s has %zu entries, need nc = %d", pk[q], n, nc);
+      napi_throw_range_error(env, NULL, buf);
+      return NULL;
+    }
   }
   const char *ck[4] = {"CPsi0", "CPsi1", "CPsi2", "CPsi3"};
   double *cp[4];
@@ -367,10 +389,16 @@ static napi_value RunBatch(napi_env env, napi_callback_info info) {
   if (!get_args(env, info, 5, argv)) return NULL;
   FdtdHipContext *c = handle(env, argv[0]);
   if (!c || !get_int(env, argv[1], &ts) || !get_int(env, argv[2], &count)) return NULL;
-  double *sv = typed(env, argv[3], napi_float64_array, NULL, 1, &ok);
+  size_t nv, ni;
+  double *sv = typed(env, argv[3], napi_float64_array, &nv, 1, &ok);
   if (!ok) return NULL;
-  double *si = typed(env, argv[4], napi_float64_array, NULL, 1, &ok);
+  double *si = typed(env, argv[4], napi_float64_array, &ni, 1, &ok);
   if (!ok) return NULL;
+  FdtdHipCounts k;
+  if (!counts_of(env, c, &k, "runBatch")) return NULL;
+  if ((sv && !need(env, nv, k.nSampledVoltages, count, "runBatch: voltage traces")) ||
+      (si && !need(env, ni, k.nSampledCurrents, count, "runBatch: current traces")))
+    return NULL;
   if (fdtd_hip_run_batch(c, ts, count, sv, si) != FDTD_HIP_OK) return throw_lib(env, "runBatch");
   return undefined(env);
 }
@@ -452,6 +480,14 @@ static napi_value Directivity(napi_env env, napi_callback_info info) {
   if (!ok) return NULL;
   double *dp = typed(env, argv[6], napi_float64_array, &nq, 0, &ok);
   if (!ok) return NULL;
+  FdtdHipCounts k;
+  if (!counts_of(env, c, &k, "directivity")) return NULL;
+  if (nf != (size_t)k.nFarfieldFreq) {
+    char buf[160];
+    snprintf(buf, sizeof buf, "directivity: %zu frequencies, the far-field box has %d", nf, k.nFarfieldFreq);
+    napi_throw_range_error(env, NULL, buf);
+    return NULL;
+  }
   if (na != nb || nt != nq || nt != nf * na || (prad && np != nf)) {
     napi_throw_range_error(env, NULL, "directivity: array lengths do not match nFreq x nAngles");
     return NULL;
@@ -582,10 +618,16 @@ static napi_value ReadTraces(napi_env env, napi_callback_info info) {
   if (!get_args(env, info, 5, argv)) return NULL;
   FdtdHipContext *c = handle(env, argv[0]);
   if (!c || !get_int(env, argv[1], &ts) || !get_int(env, argv[2], &count)) return NULL;
-  double *sv = typed(env, argv[3], napi_float64_array, NULL, 1, &ok);
+  size_t nv, ni;
+  double *sv = typed(env, argv[3], napi_float64_array, &nv, 1, &ok);
   if (!ok) return NULL;
-  double *si = typed(env, argv[4], napi_float64_array, NULL, 1, &ok);
+  double *si = typed(env, argv[4], napi_float64_array, &ni, 1, &ok);
   if (!ok) return NULL;
+  FdtdHipCounts k;
+  if (!counts_of(env, c, &k, "readTraces")) return NULL;
+  if ((sv && !need(env, nv, k.nSampledVoltages, count, "readTraces: voltage traces")) ||
+      (si && !need(env, ni, k.nSampledCurrents, count, "readTraces: current traces")))
+    return NULL;
   if (fdtd_hip_read_traces(c, ts, count, sv, si) != FDTD_HIP_OK) return throw_lib(env, "readTraces");
   return undefined(env);
 }
@@ -609,8 +651,11 @@ static napi_value ReadProbes(napi_env env, napi_callback_info info) {
   if (!get_args(env, info, 4, argv)) return NULL;
   FdtdHipContext *c = handle(env, argv[0]);
   if (!c || !get_int(env, argv[1], &ts) || !get_int(env, argv[2], &count)) return NULL;
-  double *out = typed(env, argv[3], napi_float64_array, NULL, 0, &ok);
+  size_t n;
+  double *out = typed(env, argv[3], napi_float64_array, &n, 0, &ok);
   if (!ok) return NULL;
+  FdtdHipCounts k;
+  if (!counts_of(env, c, &k, "readProbes") || !need(env, n, k.nProbes, count, "readProbes")) return NULL;
   if (fdtd_hip_read_probes(c, ts, count, out) != FDTD_HIP_OK) return throw_lib(env, "readProbes");
   return undefined(env);
 }

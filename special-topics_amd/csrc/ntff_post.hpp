@@ -80,6 +80,21 @@ __global__ void __launch_bounds__(256) k_ntff_power(NtffPost p, double *out) {
   if (threadIdx.x == 0) out[f] = 0.5 * acc[0];
 }
 
+// Wave numbers k = 2 pi f / c and the angle table [nA][4] = sin th, cos th, sin ph,
+// cos ph that k_ntff_directivity takes.
+__global__ void __launch_bounds__(256) k_directivity_prep(const double *freq, const double *theta, const double *phi,
+                                                          int nf, int nA, double *kwave, double *ang) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < nf) kwave[t] = 2 * M_PI * freq[t] / 2.99792458e8;
+  if (t < nA) {
+    double s, c;
+    sincos(theta[t], &s, &c);
+    ang[4 * t] = s; ang[4 * t + 1] = c;
+    sincos(phi[t], &s, &c);
+    ang[4 * t + 2] = s; ang[4 * t + 3] = c;
+  }
+}
+
 // One block per (frequency, group of NA angles).
 template <int NA>
 __global__ void __launch_bounds__(256) k_ntff_directivity(NtffPost p, const double *kwave, const double *prad,

@@ -34,7 +34,7 @@ class HipSim:
     """One problem resident on the GPU."""
 
     def __init__(self, problem: Problem, precision: str = "fp64", boundary: str = "reference",
-                 port_dft_freqs: np.ndarray | None = None, farfield: bool = True):
+                 port_dft_freqs: np.ndarray | None = None, farfield: bool = True, port_dft_first: bool = False):
         self.L = native.lib()
         self.problem = problem
         g = problem.grid
@@ -47,13 +47,18 @@ class HipSim:
         native.check(self.L.fdtd_hip_init(C.byref(dims), C.byref(h)), "init")
         self.h = h
         try:
-            self._upload(problem, port_dft_freqs, farfield)
+            self._upload(problem, port_dft_freqs, farfield, port_dft_first)
         except Exception:
             self.close()
             raise
 
-    def _upload(self, P: Problem, port_dft_freqs, farfield: bool):
+    def _upload(self, P: Problem, port_dft_freqs, farfield: bool, port_dft_first: bool = False):
         L, h = self.L, self.h
+        self.ndft = 0
+        if port_dft_first and port_dft_freqs is not None and len(port_dft_freqs) > 0:
+            fr = np.ascontiguousarray(port_dft_freqs, dtype=np.float64)
+            native.check(L.fdtd_hip_upload_port_dft(h, fr.size, _d(fr)), "upload_port_dft")
+            self.ndft = fr.size
         for w, name in enumerate(COEFF_ORDER):
             a = P.arrays["coeff/" + name]
             if a.dtype == np.float32:
@@ -106,8 +111,7 @@ class HipSim:
                              [ff["ni_ff"] * ff["nj_ff"]] * 8)
         self.n_sv = len(P.header["svoltages"])
         self.n_si = len(P.header["scurrents"])
-        self.ndft = 0
-        if port_dft_freqs is not None and len(port_dft_freqs) > 0:
+        if not port_dft_first and port_dft_freqs is not None and len(port_dft_freqs) > 0:
             fr = np.ascontiguousarray(port_dft_freqs, dtype=np.float64)
             native.check(L.fdtd_hip_upload_port_dft(h, fr.size, _d(fr)), "upload_port_dft")
             self.ndft = fr.size

@@ -6,7 +6,10 @@
  * from multi_run_async instead of touching freed memory.  Every call appends a token
  * to an event log, which is written as JSON to $FDTD_HIP_STUB_REPORT when the process
  * exits: i<k> init of slot k, d<k> destroy of slot k, C batch create, R batch run,
- * X batch run refused, D batch destroy, M device_count (a marker the driver places).
+ * X batch run refused, D batch destroy, M device_count (a marker the driver places),
+ * L<name> a call that takes caller arrays reached the library.  Every context reports
+ * fixed object counts (fdtd_hip_get_counts: 1 sampled voltage, 2 sampled currents,
+ * 3 probes, 2 NTFF frequencies), so a driver can check the addon's length checks.
  * Used only by tests/test_addon_handles.py, which builds the addon against it.
  */
 #include <stdio.h>
@@ -90,6 +93,48 @@ void fdtd_hip_multi_destroy(FdtdHipMulti *m) {
   if (!m) return;
   m->alive = 0;
   event("D", -1);
+}
+
+int fdtd_hip_get_counts(FdtdHipContext *c, FdtdHipCounts *out) {
+  (void)c;
+  out->nSampledVoltages = 1; out->nSampledCurrents = 2; out->nProbes = 3;
+  out->nSources = 1; out->nFarfieldFreq = 2; out->nPortDftFreq = 0;
+  return FDTD_HIP_OK;
+}
+
+int fdtd_hip_run_batch(FdtdHipContext *c, int ts, int n, double *sv, double *si) {
+  (void)c; (void)ts; (void)n; (void)sv; (void)si;
+  event("Lrun", -1);
+  return FDTD_HIP_OK;
+}
+
+int fdtd_hip_read_traces(FdtdHipContext *c, int ts, int n, double *sv, double *si) {
+  (void)c; (void)ts; (void)n; (void)sv; (void)si;
+  event("Ltraces", -1);
+  return FDTD_HIP_OK;
+}
+
+int fdtd_hip_read_probes(FdtdHipContext *c, int ts, int n, double *out) {
+  (void)c; (void)ts; (void)n; (void)out;
+  event("Lprobes", -1);
+  return FDTD_HIP_OK;
+}
+
+int fdtd_hip_upload_cpml(FdtdHipContext *c, int faceId, int nc, const double *b_e, const double *a_e,
+                         const double *b_m, const double *a_m, const double *CPsi0, long long n0,
+                         const double *CPsi1, long long n1, const double *CPsi2, long long n2,
+                         const double *CPsi3, long long n3, int m_start, int e_start) {
+  (void)c; (void)faceId; (void)nc; (void)b_e; (void)a_e; (void)b_m; (void)a_m; (void)CPsi0; (void)n0;
+  (void)CPsi1; (void)n1; (void)CPsi2; (void)n2; (void)CPsi3; (void)n3; (void)m_start; (void)e_start;
+  event("Lcpml", -1);
+  return FDTD_HIP_OK;
+}
+
+int fdtd_hip_directivity(FdtdHipContext *c, const double *freqHz, const double *radPower, int nAngles,
+                         const double *theta, const double *phi, double *dataTheta, double *dataPhi) {
+  (void)c; (void)freqHz; (void)radPower; (void)nAngles; (void)theta; (void)phi; (void)dataTheta; (void)dataPhi;
+  event("Ldir", -1);
+  return FDTD_HIP_OK;
 }
 
 __attribute__((destructor)) static void report(void) {

@@ -40,7 +40,7 @@ def _check_record(rec):
     assert rec["runAfterCollect"] is None, rec
 
 
-def test_handles_against_stub(tmp_path):
+def _build_against_stub(tmp_path):
     lib = tmp_path / "libfdtd_hip.so"
     (tmp_path / "js").mkdir()
     node_mod = tmp_path / "js" / "fdtd_hip.node"
@@ -51,6 +51,11 @@ def test_handles_against_stub(tmp_path):
                           ADDON_SRC, f"-L{tmp_path}", "-l:libfdtd_hip.so", "-Wl,-rpath,$ORIGIN/..",
                           "-Wl,--allow-shlib-undefined"], capture_output=True, text=True)
     assert res.returncode == 0, res.stderr[-3000:]
+    return node_mod
+
+
+def test_handles_against_stub(tmp_path):
+    node_mod = _build_against_stub(tmp_path)
     report = tmp_path / "report.json"
     env = dict(os.environ, FDTD_HIP_STUB_REPORT=str(report))
     rec = _run_driver(str(node_mod), tmp_path / "out.json", env=env)
@@ -69,3 +74,25 @@ def test_handles_against_the_library(tmp_path):
     from fdtdhip import native
     rec = _run_driver(native.ADDON_PATH, tmp_path / "out.json", mode="live")
     _check_record(rec)
+
+
+def test_caller_array_lengths_are_checked(tmp_path):
+    """Arrays shorter than the library would write or read are refused with a
+    RangeError before the call reaches it: trace and probe buffers (count steps per
+    observer), CPML profiles (nc entries), the directivity frequency list (the NTFF
+    box's frequencies).  Arrays of the right size go through."""
+    node_mod = _build_against_stub(tmp_path)
+    report = tmp_path / "report.json"
+    out = tmp_path / "out.json"
+    res = subprocess.run(["node", os.path.join(REPO, "tests", "js", "lengths_driver.mjs"), str(node_mod), str(out)],
+                         capture_output=True, text=True, timeout=120,
+                         env=dict(os.environ, FDTD_HIP_STUB_REPORT=str(report)))
+    assert res.returncode == 0, res.stderr[-3000:]
+    rec = json.loads(out.read_text())
+    for key in ("runShortV", "runShortI", "tracesShort", "probesShort", "cpmlShort", "dirFreq"):
+        assert rec[key] and rec[key].startswith("RangeError"), (key, rec[key])
+    for key in ("runOk", "runNull", "tracesOk", "probesOk", "cpmlOk", "dirOk"):
+        assert rec[key] is None, (key, rec[key])
+    rep = json.loads(report.read_text())
+    assert [t for t in rep["events"].split() if t.startswith("L")] == \
+        "Lrun Lrun Ltraces Lprobes Lcpml Ldir".split(), rep["events"]

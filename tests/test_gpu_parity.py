@@ -270,6 +270,23 @@ def test_port_running_dft_matches_trace_dft(get_problem):
         assert np.max(np.abs(dft[o] - ref)) <= 1e-12 * scale
 
 
+def test_port_dft_upload_order_does_not_matter(get_problem):
+    """The running port DFT may be uploaded before the observers: its accumulators are
+    sized at the first step from every observer uploaded by then, so both orders give
+    the same transforms bit for bit, and reading them before any step gives zeros."""
+    prob = get_problem("golden-small")
+    freqs = np.arange(1, 41) * 0.5e9
+    out = []
+    for first in (False, True):
+        sim = _sim(prob, port_dft_freqs=freqs, port_dft_first=first)
+        assert not np.any(sim.port_dft())
+        sim.run(0, prob.steps)
+        out.append(sim.port_dft())
+        sim.close()
+    assert out[0].shape == (4, freqs.size) and np.any(out[0])
+    assert np.array_equal(out[0], out[1])
+
+
 def test_fp32_tracks_fp64(get_problem):
     prob = get_problem("golden-small")
     s64 = _sim(prob)
