@@ -206,6 +206,23 @@ int fdtd_hip_radiated_power(FdtdHipContext *ctx, double *out, int nFreq);
 int fdtd_hip_directivity(FdtdHipContext *ctx, const double *freqHz, const double *radPower, int nAngles,
                          const double *theta, const double *phi, double *dataTheta, double *dataPhi);
 
+/* fdtd_hip_directivity without any device allocation, for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  No counterpart in the reference.  Same arguments and results as
+ * fdtd_hip_directivity (identical bits: the same kernels run), but the staged inputs,
+ * the radiated power and the outputs live in a fixed per-device workspace of the
+ * library (about 1.4 MB, held until the process ends), so nFreq (the NTFF box's
+ * frequencies) <= FDTD_HIP_NOALLOC_MAX_FREQ, nAngles <= FDTD_HIP_NOALLOC_MAX_ANGLES
+ * and nFreq * nAngles <= FDTD_HIP_NOALLOC_MAX_OUTPUTS; larger requests fail with an
+ * error.  Every call on one device shares that workspace: calls that overlap on one
+ * device (from several threads or contexts) race, so serialise them.  A call returns
+ * when its results are on the host. */
+#define FDTD_HIP_NOALLOC_MAX_FREQ 64
+#define FDTD_HIP_NOALLOC_MAX_ANGLES 8192
+#define FDTD_HIP_NOALLOC_MAX_OUTPUTS 65536
+int fdtd_hip_directivity_noalloc(FdtdHipContext *ctx, const double *freqHz, const double *radPower, int nAngles,
+                                 const double *theta, const double *phi, double *dataTheta, double *dataPhi);
+
 /* How the NTFF running DFT is accumulated.  DIRECT: every accumulator of every
  * frequency is updated on every step in the reference's exact arithmetic
  * (farfield.js:90-248; bit-identical in FP64).  BLOCKED: each step stores its surface

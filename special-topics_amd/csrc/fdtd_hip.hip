@@ -20,6 +20,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <unordered_map>
@@ -59,6 +60,41 @@ struct DevBuf {
   void *p = nullptr;
   size_t bytes = 0;
 };
+
+}  // namespace
+
+// Fixed device workspace of fdtd_hip_directivity_noalloc (staged inputs, wave numbers,
+// angle trigonometry, radiated power, outputs), laid out as DirLayout for the maximum
+// sizes: 2*64 + 64 + 2*8192 + 4*8192 + 2*65536 doubles, about 1.4 MB per device, held
+// until the process ends.  Host code never uses the symbol's name as an address: each
+// device's copy is found once with hipGetSymbolAddress (noalloc_register).
+constexpr long NOALLOC_DOUBLES =
+    3L * FDTD_HIP_NOALLOC_MAX_FREQ + 6L * FDTD_HIP_NOALLOC_MAX_ANGLES + 2L * FDTD_HIP_NOALLOC_MAX_OUTPUTS;
+static __device__ __align__(256) double scratch[NOALLOC_DOUBLES];
+
+namespace {
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex g_noalloc_mu;
+double *g_noalloc_ptr[NOALLOC_MAX_DEVICES] = {};
+
+// Device address of `scratch` on `device` (current), looked up once per device.
+int noalloc_register(int device) {
+  if (device < 0 || device >= NOALLOC_MAX_DEVICES) return fail("init: device %d out of range", device);
+  std::lock_guard<std::mutex> lock(g_noalloc_mu);
+  if (g_noalloc_ptr[device]) return FDTD_HIP_OK;
+  void *ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !ptr) return fail("init: hipGetSymbolAddress(scratch) failed: %s", hipGetErrorString(e));
+  g_noalloc_ptr[device] = (double *)ptr;
+  return FDTD_HIP_OK;
+}
+
+double *noalloc_workspace(int device) {
+  if (device < 0 || device >= NOALLOC_MAX_DEVICES) return nullptr;
+  std::lock_guard<std::mutex> lock(g_noalloc_mu);
+  return g_noalloc_ptr[device];
+}
 
 }  // namespace
 
@@ -767,8 +803,8 @@ int fdtd_hip_init(const FdtdHipDims *dims, FdtdHipContext **out) {
   // Each uniform array: guard plane, nx+1 planes, guard plane.
   c->stride_bytes = ((size_t)(d.nx + 3) * c->SI * c->esz + 255) / 256 * 256;
   if (hipGetDevice(&c->device) != hipSuccess) c->device = 0;
-  int rc = FDTD_HIP_OK;
-  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) rc = fail("init: stream creation failed");
+  int rc = noalloc_register(c->device);
+  if (!rc && hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) rc = fail("init: stream creation failed");
   if (!rc) rc = dev_alloc(c, &c->fields, 6 * c->stride_bytes);
   if (!rc) rc = dev_alloc(c, &c->coeffs, NCOEFF * c->stride_bytes);
   if (!rc) rc = dev_alloc(c, (void **)&c->d_step, 64);
@@ -1219,7 +1255,8 @@ int fdtd_hip_radiated_power(FdtdHipContext *c, double *out, int nFreq) {
 // Directivity of a context into device workspace `ws` laid out by DirLayout for
 // capacities (nf_cap, na_cap): inputs staged from the host, the trigonometry of the
 // angles and the wave numbers on the device, the radiated power (unless given) and the
-// directivity kernel, outputs copied back.
+// directivity kernel, outputs copied back.  Shared by the allocating and the
+// no-allocation entry points, which therefore give identical results.
 struct DirLayout {
   long kw, prad, freq, theta, phi, ang, dth, dph, total;
   DirLayout(long nf_cap, long na_cap, long out_cap) {
@@ -1268,6 +1305,20 @@ int fdtd_hip_directivity(FdtdHipContext *c, const double *freqHz, const double *
   Scratch s;
   double *ws = s.put<double>(c, nullptr, (size_t)L.total);
   if (!ws) return fail("directivity: device allocation failed");
+  return directivity_impl(c, freqHz, radPower, nAngles, theta, phi, dataTheta, dataPhi, ws, L);
+}
+
+int fdtd_hip_directivity_noalloc(FdtdHipContext *c, const double *freqHz, const double *radPower, int nAngles,
+                                 const double *theta, const double *phi, double *dataTheta, double *dataPhi) {
+  if (directivity_check(c, freqHz, theta, phi, dataTheta, dataPhi, nAngles, "directivity_noalloc")) return FDTD_HIP_ERR;
+  if (c->nff > FDTD_HIP_NOALLOC_MAX_FREQ || nAngles > FDTD_HIP_NOALLOC_MAX_ANGLES ||
+      (long)c->nff * nAngles > FDTD_HIP_NOALLOC_MAX_OUTPUTS)
+    return fail("directivity_noalloc: %d frequencies x %d angles exceed the fixed workspace (at most %d frequencies, "
+                "%d angles, %d outputs); use fdtd_hip_directivity", c->nff, nAngles, FDTD_HIP_NOALLOC_MAX_FREQ,
+                FDTD_HIP_NOALLOC_MAX_ANGLES, FDTD_HIP_NOALLOC_MAX_OUTPUTS);
+  double *ws = noalloc_workspace(c->device);
+  if (!ws) return fail("directivity_noalloc: no workspace address for device %d", c->device);
+  const DirLayout L(FDTD_HIP_NOALLOC_MAX_FREQ, FDTD_HIP_NOALLOC_MAX_ANGLES, FDTD_HIP_NOALLOC_MAX_OUTPUTS);
   return directivity_impl(c, freqHz, radPower, nAngles, theta, phi, dataTheta, dataPhi, ws, L);
 }
 

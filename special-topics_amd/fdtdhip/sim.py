@@ -215,6 +215,18 @@ class HipSim:
                      "directivity")
         return dt, dp
 
+    def directivity_noalloc(self, theta, phi, rad_power=None):
+        """As directivity(), through the entry point that allocates no device memory."""
+        th = np.ascontiguousarray(theta, dtype=np.float64)
+        ph = np.ascontiguousarray(phi, dtype=np.float64)
+        fr = np.ascontiguousarray(self.problem.header["ff"]["frequencies"], dtype=np.float64)
+        dt = np.zeros((self.nfreq, th.size))
+        dp = np.zeros((self.nfreq, th.size))
+        rp = None if rad_power is None else _d(np.ascontiguousarray(rad_power, dtype=np.float64))
+        native.check(self.L.fdtd_hip_directivity_noalloc(self.h, _d(fr), rp, th.size, _d(th), _d(ph), _d(dt),
+                                                         _d(dp)), "directivity_noalloc")
+        return dt, dp
+
     def port_dft(self):
         n = (self.n_sv + self.n_si) * self.ndft
         re, im = np.zeros(n), np.zeros(n)
