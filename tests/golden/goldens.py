@@ -30,6 +30,10 @@ def f64(b64: str) -> np.ndarray:
     return np.frombuffer(base64.b64decode(b64), dtype=np.float64)
 
 
+def b64(a: np.ndarray) -> str:
+    return base64.b64encode(np.ascontiguousarray(a, dtype="<f8").tobytes()).decode("ascii")
+
+
 def sha(a: np.ndarray) -> str:
     return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
 
