@@ -11,12 +11,16 @@ strip plus a PEC plate through the substrate) at the arm's end.
 Per variant, drawn from numpy's PCG64 seeded with (seed, index):
   arm length L ~ U[10,30] mm, arm height above the ground edge h ~ U[3,8] mm,
   feed-to-short spacing s ~ U[2,10] mm (clamped to L - 5 mm so the feed stays on the
-  arm), ground plane Gw x Gl ~ U[30,60] x U[80,120] mm;
-all snapped to the 1 mm in-plane cell.  Cells: dx = 0.262 mm (3 across the substrate),
-dy = dz = 1 mm; 10 air cells and an 8-cell CPML on every face, so grids are
-39 x (73..108) x (116..156).  Derivative-Gaussian pulse to 6 GHz, 7000 steps at
-0.95 x Courant, S11 at 1000 frequencies (6 MHz .. 6 GHz), NTFF at 16 frequencies
-1.5 .. 6.0 GHz (the far field is reported at the one nearest the S11 resonance).
+  arm), ground plane Gw x Gl ~ U[30,60] x U[80,120] mm, and the grid's x extent
+  nx ~ U{32..48} cells;
+y lengths snapped to the 0.6 mm y cell, z lengths to the 1 mm z cell.  Cells:
+dx = 0.262 mm (3 across the substrate), dy = 0.6 mm, dz = 1 mm, an 8-cell CPML on every
+face; the air between antenna and CPML is 10 cells in z, at least 10 in y (more for
+narrow boards, so ny >= 110) and in x whatever makes nx the drawn value (6..15 cells a
+side).  Grids are therefore 32..48 x 110..159 x 116..156.  Derivative-Gaussian pulse to
+6 GHz, 7000 steps at 0.95 x Courant, S11 at 1000 frequencies (6 MHz .. 6 GHz), NTFF at
+16 frequencies 1.5 .. 6.0 GHz (the far field is reported at the one nearest the S11
+resonance).
 """
 
 from __future__ import annotations
@@ -27,18 +31,20 @@ import os
 
 import numpy as np
 
-DX, DYZ = 0.262e-3, 1e-3
+DX, DY, DZ = 0.262e-3, 0.6e-3, 1e-3
 T_SUB = 3 * DX            # substrate thickness
+CPML = 8
+NY_MIN = 110
 W = 2e-3                  # trace, arm and stub width
 NTFF_FREQS = [1.5e9 + 0.3e9 * q for q in range(16)]
 
 
-def boundary() -> dict:
+def boundary(air: dict) -> dict:
     b = {}
     for f in ("xn", "xp", "yn", "yp", "zn", "zp"):
         b["type_" + f] = "cpml"
-        b["air_buffer_" + f] = 10
-        b["cpml_cells_" + f] = 8
+        b["air_buffer_" + f] = air[f]
+        b["cpml_cells_" + f] = CPML
     b.update({"cpml_order": 3, "cpml_sigma_factor": 1.3, "cpml_kappa_max": 7,
               "cpml_alpha_min": 0, "cpml_alpha_max": 0.05})
     return b
@@ -46,11 +52,20 @@ def boundary() -> dict:
 
 def draw(seed: int, index: int) -> dict:
     rng = np.random.Generator(np.random.PCG64([seed, index]))
-    mm = lambda lo, hi: round(float(rng.uniform(lo, hi))) * 1e-3
-    p = {"arm_length": mm(10, 30), "arm_height": mm(3, 8), "feed_short": mm(2, 10),
-         "ground_w": mm(30, 60), "ground_l": mm(80, 120)}
+    u = lambda lo, hi, d: round(float(rng.uniform(lo, hi)) * 1e-3 / d) * d
+    p = {"arm_length": u(10, 30, DZ), "arm_height": u(3, 8, DY), "feed_short": u(2, 10, DZ),
+         "ground_w": u(30, 60, DY), "ground_l": u(80, 120, DZ)}
     p["feed_short"] = min(p["feed_short"], p["arm_length"] - 5e-3)
+    p["nx"] = int(rng.integers(32, 49))
     return p
+
+
+def air_buffers(p: dict, top: float) -> dict:
+    """Air cells per face: x sized to the drawn nx, y widened for narrow boards."""
+    ax = p["nx"] - 3 - 2 * CPML
+    cells_y = round((top + 2e-3) / DY)
+    ay = max(10, -(-(NY_MIN - 2 * CPML - cells_y) // 2))
+    return {"xn": ax // 2, "xp": ax - ax // 2, "yn": ay, "yp": ay, "zn": 10, "zp": 10}
 
 
 def scenario(seed: int, index: int, steps: int = 7000) -> dict:
@@ -58,7 +73,7 @@ def scenario(seed: int, index: int, steps: int = 7000) -> dict:
     L, h, s = p["arm_length"], p["arm_height"], p["feed_short"]
     gw, gl = p["ground_w"], p["ground_l"]
     r = lambda v: round(v, 7)                  # keep coordinates exact multiples in JSON
-    z0 = round((gl - L) / 2 / DYZ) * DYZ       # arm start (mm grid)
+    z0 = round((gl - L) / 2 / DZ) * DZ         # arm start (z grid)
     z_short = z0 + L - W                       # stub at the arm's far end
     z_feed = z_short - s - W                   # feed: s between feed and stub edges
     y_arm = gw + h
@@ -81,19 +96,19 @@ def scenario(seed: int, index: int, steps: int = 7000) -> dict:
     port = {"min_x": r(-T_SUB), "min_y": 0, "min_z": r(z_feed), "max_x": 0, "max_y": 0, "max_z": r(z_feed + W)}
     return {
         "name": f"ifa-variant-{seed}-{index}",
-        "description": "Seeded IFA variant: arm %.0f mm, height %.0f mm, feed-to-short %.0f mm, ground %.0f x %.0f mm."
-                       % (L * 1e3, h * 1e3, s * 1e3, gw * 1e3, gl * 1e3),
+        "description": "Seeded IFA variant: arm %.1f mm, height %.1f mm, feed-to-short %.1f mm, ground %.1f x %.1f mm, "
+                       "nx %d." % (L * 1e3, h * 1e3, s * 1e3, gw * 1e3, gl * 1e3, p["nx"]),
         "parameters": {k: v for k, v in p.items()},
         "numberOfTimeSteps": steps, "courantFactor": 0.95, "numberOfCellsPerWavelength": 20,
-        "cell": {"dx": DX, "dy": DYZ, "dz": DYZ},
-        "boundary": boundary(),
+        "cell": {"dx": DX, "dy": DY, "dz": DZ},
+        "boundary": boundary(air_buffers(p, top)),
         "materialTypes": mats, "bricks": bricks, "spheres": [],
         "waveforms": {"derivative_gaussian": [{"number_of_cells_per_wavelength": 50}]},
         "voltageSources": [dict(port, direction="xp", resistance=50, magnitude=1,
                                 waveform_type="derivative_gaussian", waveform_index=1)],
         "sampledVoltages": [dict(port, direction="xp", label="v1")],
         "sampledCurrents": [{"min_x": -0.00039, "min_y": 0, "min_z": r(z_feed), "max_x": -0.00039,
-                             "max_y": DYZ, "max_z": r(z_feed + W), "direction": "xp", "label": "i1"}],
+                             "max_y": DY, "max_z": r(z_feed + W), "direction": "xp", "label": "i1"}],
         "ports": [{"sampled_voltage_index": 1, "sampled_current_index": 1, "impedance": 50, "is_source_port": True}],
         "farfield": {"frequencies": NTFF_FREQS, "number_of_cells_from_outer_boundary": 13},
         "dft": {"maxFreq": 6e9, "step": 6e6},
