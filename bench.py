@@ -333,7 +333,7 @@ def stencil_study(workload, precision, steps, warmup, timing_steps, workdir, ntf
     nh, ne = psi_counts(problem)
     cb = cell_phase_bytes(esz, coef_mode)
     dom = 0 if avg_ms[0] >= avg_ms[1] else 1
-    fused_axes = {0: "", 1: "z", 2: "xyz"}[tuning[3]]
+    fused_axes = {0: "", 1: "z", 2: "xyz", 3: "z"}[tuning[3]]
     fh, fe = psi_counts(problem, fused_axes) if fused_axes else (0, 0)
     psi_fused = PSI_VALUES * esz * (fh if dom == 0 else fe)
     dom_bytes = cb * ncells + psi_fused
@@ -342,14 +342,18 @@ def stencil_study(workload, precision, steps, warmup, timing_steps, workdir, ntf
     achieved = dom_bytes / dom_s / 1e9
 
     ctype = "float" if fp32 else "double"
-    kname = f"fdtd::k_update_{'h' if dom == 0 else 'e'}<{ctype}, {coef_mode}, {max(tuning[1], 1)}, {tuning[3]}>"
+    def stencil_name(phase):
+        w = max(tuning[1], 1)
+        if tuning[3] == 3:          # z slabs in workgroups of their own: the split kernels
+            return f"fdtd::k_update_{phase}_split<{ctype}, {coef_mode}, {w}>"
+        return f"fdtd::k_update_{phase}<{ctype}, {coef_mode}, {w}, {tuning[3]}>"
+    kname = stencil_name("h" if dom == 0 else "e")
     table = traffic_table(workload, precision)
     traffic = table.get(kname)
     step_s = elapsed / steps
     # Whole step, counted: the two stencils, the two CPML passes and the small E-tail
     # kernels as the timing steps launch them (separately).
-    step_kernels = [f"fdtd::k_update_h<{ctype}, {coef_mode}, {max(tuning[1], 1)}, {tuning[3]}>",
-                    f"fdtd::k_update_e<{ctype}, {coef_mode}, {max(tuning[1], 1)}, {tuning[3]}>"]
+    step_kernels = [stencil_name("h"), stencil_name("e")]
     if tuning[3] != 2:
         step_kernels += [f"fdtd::k_cpml_h<{ctype}>", f"fdtd::k_cpml_e<{ctype}>"]
     step_traffic = sum(table[k] for k in step_kernels) if all(k in table for k in step_kernels) else None
@@ -389,7 +393,8 @@ def stencil_study(workload, precision, steps, warmup, timing_steps, workdir, ntf
         "setup_s": round(setup_s, 2),
         "tuning": {"planes_per_workgroup": tuning[0], "cells_per_thread": tuning[1],
                    "coefficient_storage": ["indexed", "arrays"][tuning[2]],
-                   "cpml": ["separate pass", "z slabs fused into bulk", "all faces fused into bulk"][tuning[3]]},
+                   "cpml": ["separate pass", "z slabs fused into bulk", "all faces fused into bulk",
+                            "z slabs in their own workgroups of the bulk launch"][tuning[3]]},
     }
 
 
@@ -430,7 +435,8 @@ def main():
     ap.add_argument("--cells", type=int, default=0, help="cells per thread (with --planes)")
     ap.add_argument("--coef", type=int, default=0, help="coefficient storage (with --planes)")
     ap.add_argument("--fuse-cpml", type=int, default=0,
-                    help="CPML placement with --planes: 0 separate pass, 1 z slabs fused, 2 all faces fused")
+                    help="CPML placement with --planes: 0 separate pass, 1 z slabs fused, 2 all faces fused, "
+                         "3 z slabs in workgroups of their own")
     ap.add_argument("--pmc-summary", nargs=3, metavar=("FETCH_DIR", "WRITE_DIR", "OUT_JSON"),
                     help="write the calibrated per-kernel traffic summary of two rocprofv3 --pmc runs and exit")
     args = ap.parse_args()

@@ -22,8 +22,14 @@
 namespace fdtd {
 
 // CPML placement (a tuning choice; results are identical): the separate k_cpml pass
-// for every face, the z-face slabs fused into the bulk kernels, or every face fused.
-constexpr int CPML_SEPARATE = 0, CPML_FUSE_Z = 1, CPML_FUSE_ALL = 2;
+// for every face, the z-face slabs fused into the bulk kernels, every face fused, or
+// the z-face slab cells (bulk update and every CPML term) in workgroups of their own
+// within the bulk kernels' launch -- FUSE_Z's split of the work, without z-slab cells
+// in the bulk kernels' waves.  SPLIT_Z leaves the CPML pass the same cells as FUSE_Z.
+constexpr int CPML_SEPARATE = 0, CPML_FUSE_Z = 1, CPML_FUSE_ALL = 2, CPML_SPLIT_Z = 3;
+
+// The tile list (build_cpml_tiles mode) the CPML pass uses under a placement.
+constexpr int cpml_tile_mode(int placement) { return placement == CPML_SPLIT_Z ? CPML_FUSE_Z : placement; }
 
 struct CpmlTile {
   unsigned x, y, z, w;
@@ -146,6 +152,60 @@ inline void build_cpml_tiles(const CpmlLayout &L, int phase, int mode, std::vect
         for (const auto &c : chunks)
           if (q < c.size()) out.insert(out.end(), c[q].begin(), c[q].end());
     }
+}
+
+// Lanes of the z-slab march (CPML_SPLIT_Z) of one phase: the cells (j, k) of the two
+// z-face slabs over rows 0..rows-1, as segments of hi_n + lo_n lanes: segment s holds
+// row s-1's high-z slab cells, then row s's low-z slab cells, which lie a few dozen
+// bytes apart in the field arrays.  Workgroups of 256 lanes march `chunk` x planes;
+// tiles are numbered lane-group fastest and dealt to the 8 XCDs in contiguous ranges,
+// like the bulk kernels' tiles.  ntiles is a multiple of 8.
+struct ZslabGeom {
+  int lo_start, lo_n, hi_start, hi_n;   // slab k ranges (n = 0 when the face is absent)
+  int rows, lanes;                      // rows 0..rows-1; (rows + 1) * (lo_n + hi_n) lanes
+  int ncol, chunk, nchunk, per_xcd, ntiles;
+};
+
+// The (j, k) of lane q; false for lanes past the rows.
+constexpr bool zslab_lane(const ZslabGeom &z, int q, int &j, int &k) {
+  const int per = z.lo_n + z.hi_n;
+  if (per == 0 || q >= z.lanes) return false;
+  const int seg = q / per, w = q - seg * per;
+  if (w < z.hi_n) { j = seg - 1; k = z.hi_start + w; }
+  else { j = seg; k = z.lo_start + (w - z.hi_n); }
+  return j >= 0 && j < z.rows;
+}
+
+// The march tile (lane group, x chunk) workgroup L handles; false past the tiles.
+constexpr bool zslab_tile(const ZslabGeom &z, int L, int &col, int &ch) {
+  const int t = (L & 7) * z.per_xcd + (L >> 3);
+  if (t >= z.ncol * z.nchunk) return false;
+  col = t % z.ncol;
+  ch = t / z.ncol;
+  return true;
+}
+
+// Z-slab march geometry of phase (0 = H, 1 = E) over `planes` x planes.
+inline ZslabGeom zslab_geom(const CpmlLayout &L, int phase, int planes, int chunk) {
+  ZslabGeom z{};
+  const int lo = 4, hi = 5;
+  z.lo_n = L.nc[lo];
+  z.lo_start = phase == 0 ? L.m_start[lo] : L.e_start[lo];
+  z.hi_n = L.nc[hi];
+  z.hi_start = phase == 0 ? L.m_start[hi] : L.e_start[hi];
+  if (z.lo_n && z.hi_n && z.lo_start <= z.hi_start + z.hi_n && z.hi_start <= z.lo_start + z.lo_n) {
+    // Overlapping or touching slabs: one range, so no cell is marched twice.
+    const int a = std::min(z.lo_start, z.hi_start), b = std::max(z.lo_start + z.lo_n, z.hi_start + z.hi_n);
+    z.lo_start = a; z.lo_n = b - a; z.hi_n = 0;
+  }
+  z.rows = L.n[1] + 1;
+  z.lanes = (z.rows + 1) * (z.lo_n + z.hi_n);
+  z.ncol = (z.lanes + 255) / 256;
+  z.chunk = chunk > 0 ? chunk : 1;
+  z.nchunk = (planes + z.chunk - 1) / z.chunk;
+  z.per_xcd = (z.ncol * z.nchunk + 7) / 8;
+  z.ntiles = z.lanes ? 8 * z.per_xcd : 0;
+  return z;
 }
 
 // Whether the E CPML pass may change E component `comp` (0..2 = Ex..Ez) at (i, j, k):

@@ -486,24 +486,6 @@ void launch_e_f(FdtdHipContext *c, const Grid<T> &g, const MarchGeom &m, int gho
   else launch_e_w<T, COEF_ARRAYS, FC>(c, g, m, ghost, fs);
 }
 
-template <typename T>
-void launch_h(FdtdHipContext *c, const Grid<T> &g, const MarchGeom &m, const FaceSet<T> &fs) {
-  switch (fuse_cpml(c)) {
-    case CPML_FUSE_ALL: launch_h_f<T, CPML_FUSE_ALL>(c, g, m, fs); break;
-    case CPML_FUSE_Z: launch_h_f<T, CPML_FUSE_Z>(c, g, m, fs); break;
-    default: launch_h_f<T, CPML_SEPARATE>(c, g, m, fs); break;
-  }
-}
-
-template <typename T>
-void launch_e(FdtdHipContext *c, const Grid<T> &g, const MarchGeom &m, int ghost, const FaceSet<T> &fs) {
-  switch (fuse_cpml(c)) {
-    case CPML_FUSE_ALL: launch_e_f<T, CPML_FUSE_ALL>(c, g, m, ghost, fs); break;
-    case CPML_FUSE_Z: launch_e_f<T, CPML_FUSE_Z>(c, g, m, ghost, fs); break;
-    default: launch_e_f<T, CPML_SEPARATE>(c, g, m, ghost, fs); break;
-  }
-}
-
 // The CPML faces of a context as the tile builder takes them.
 CpmlLayout cpml_layout(const FdtdHipContext *c) {
   CpmlLayout L{};
@@ -514,6 +496,64 @@ CpmlLayout cpml_layout(const FdtdHipContext *c) {
     L.e_start[f] = c->face[f].e_start;
   }
   return L;
+}
+
+// CPML_SPLIT_Z launches: the z-slab march workgroups first, then the stencil tiles.
+// The z-slab march of H covers planes 0..nx, that of E planes 0..nx-1 (the E wall
+// rows come with plane 0).
+template <typename T, int CM>
+void launch_h_split(FdtdHipContext *c, const Grid<T> &g, const MarchGeom &m, const FaceSet<T> &fs) {
+  const ZslabGeom z = zslab_geom(cpml_layout(c), 0, c->d.nx + 1, m.chunk);
+  const dim3 grid(z.ntiles + 8 * m.per_xcd), block(256);
+  const int w = cells_per_thread(c);
+  if constexpr (sizeof(T) == 4) {
+    if (w == 4) { hipLaunchKernelGGL((k_update_h_split<T, CM, 4>), grid, block, 0, c->stream, g, m, z, fs); return; }
+  }
+  if (w == 2) hipLaunchKernelGGL((k_update_h_split<T, CM, 2>), grid, block, 0, c->stream, g, m, z, fs);
+  else hipLaunchKernelGGL((k_update_h_split<T, CM, 1>), grid, block, 0, c->stream, g, m, z, fs);
+}
+
+template <typename T, int CM>
+void launch_e_split(FdtdHipContext *c, const Grid<T> &g, const MarchGeom &m, int ghost, const FaceSet<T> &fs) {
+  const ZslabGeom z = zslab_geom(cpml_layout(c), 1, c->d.nx, m.chunk);
+  const dim3 grid(z.ntiles + 8 * m.per_xcd), block(256);
+  const int w = cells_per_thread(c);
+  if constexpr (sizeof(T) == 4) {
+    if (w == 4) {
+      hipLaunchKernelGGL((k_update_e_split<T, CM, 4>), grid, block, 0, c->stream, g, m, z, ghost, fs);
+      return;
+    }
+  }
+  if (w == 2) hipLaunchKernelGGL((k_update_e_split<T, CM, 2>), grid, block, 0, c->stream, g, m, z, ghost, fs);
+  else hipLaunchKernelGGL((k_update_e_split<T, CM, 1>), grid, block, 0, c->stream, g, m, z, ghost, fs);
+}
+
+template <typename T>
+void launch_h(FdtdHipContext *c, const Grid<T> &g, const MarchGeom &m, const FaceSet<T> &fs) {
+  switch (fuse_cpml(c)) {
+    case CPML_FUSE_ALL: launch_h_f<T, CPML_FUSE_ALL>(c, g, m, fs); break;
+    case CPML_FUSE_Z: launch_h_f<T, CPML_FUSE_Z>(c, g, m, fs); break;
+    case CPML_SPLIT_Z:
+      if (c->coef_mode == COEF_U8) launch_h_split<T, COEF_U8>(c, g, m, fs);
+      else if (c->coef_mode == COEF_U16) launch_h_split<T, COEF_U16>(c, g, m, fs);
+      else launch_h_split<T, COEF_ARRAYS>(c, g, m, fs);
+      break;
+    default: launch_h_f<T, CPML_SEPARATE>(c, g, m, fs); break;
+  }
+}
+
+template <typename T>
+void launch_e(FdtdHipContext *c, const Grid<T> &g, const MarchGeom &m, int ghost, const FaceSet<T> &fs) {
+  switch (fuse_cpml(c)) {
+    case CPML_FUSE_ALL: launch_e_f<T, CPML_FUSE_ALL>(c, g, m, ghost, fs); break;
+    case CPML_FUSE_Z: launch_e_f<T, CPML_FUSE_Z>(c, g, m, ghost, fs); break;
+    case CPML_SPLIT_Z:
+      if (c->coef_mode == COEF_U8) launch_e_split<T, COEF_U8>(c, g, m, ghost, fs);
+      else if (c->coef_mode == COEF_U16) launch_e_split<T, COEF_U16>(c, g, m, ghost, fs);
+      else launch_e_split<T, COEF_ARRAYS>(c, g, m, ghost, fs);
+      break;
+    default: launch_e_f<T, CPML_SEPARATE>(c, g, m, ghost, fs); break;
+  }
 }
 
 // Tiles of both phases for the current placement, building every placement's lists
@@ -540,7 +580,7 @@ int set_cpml_tiles(FdtdHipContext *c) {
     if (!all.empty() && upload_vec(c, c->tile_buf, all)) return FDTD_HIP_ERR;
     c->tiles_version = c->faces_version;
   }
-  const int mode = fuse_cpml(c);
+  const int mode = cpml_tile_mode(fuse_cpml(c));
   for (int ph = 0; ph < 2; ++ph) {
     c->cpml_tiles[ph] = c->tile_buf ? c->tile_buf + c->tile_off[mode][ph] : nullptr;
     c->ncpml[ph] = (long)c->tile_n[mode][ph];
@@ -1357,8 +1397,8 @@ static int check_tuning(const FdtdHipContext *c, const FdtdHipTuning *t, const c
     return fail("%s: %d cells per thread unsupported (1, 2%s)", who, t->cells_per_thread, maxw == 4 ? " or 4" : "");
   if (t->coefficient_storage < 0 || t->coefficient_storage > 1)
     return fail("%s: coefficient storage %d unknown", who, t->coefficient_storage);
-  if (t->fuse_cpml < CPML_SEPARATE || t->fuse_cpml > CPML_FUSE_ALL)
-    return fail("%s: fuse_cpml %d unknown (0, 1 or 2)", who, t->fuse_cpml);
+  if (t->fuse_cpml < CPML_SEPARATE || t->fuse_cpml > CPML_SPLIT_Z)
+    return fail("%s: fuse_cpml %d unknown (0, 1, 2 or 3)", who, t->fuse_cpml);
   return FDTD_HIP_OK;
 }
 
@@ -1419,7 +1459,7 @@ int fdtd_hip_autotune(FdtdHipContext *c, int steps_per_candidate, FdtdHipTuning 
     for (int w : ws) {
       if (w == 4 && !c->fp32()) continue;
       for (int pl : planes)
-        for (int fc = CPML_SEPARATE; fc <= CPML_FUSE_ALL; ++fc) cands.push_back(FdtdHipTuning{pl, w, cs, fc});
+        for (int fc = CPML_SEPARATE; fc <= CPML_SPLIT_Z; ++fc) cands.push_back(FdtdHipTuning{pl, w, cs, fc});
     }
   }
   double best = 1e300;
@@ -1920,6 +1960,7 @@ int fdtd_hip_multi_create(FdtdHipContext *const *ctxs, int n, const FdtdHipTunin
   if (tuning) {
     t = *tuning;
     if (check_tuning(ctxs[0], &t, "multi_create")) return FDTD_HIP_ERR;
+    if (t.fuse_cpml == CPML_SPLIT_Z) return fail("multi_create: CPML placement 3 (split z slabs) is single-problem only");
   }
   HIP_TRY(hipSetDevice(ctxs[0]->device));
   FdtdHipMulti *m = new FdtdHipMulti();

@@ -261,13 +261,28 @@ struct MarchGeom {
   int per_xcd;    // tiles per XCD
 };
 
-__device__ __forceinline__ bool march_tile(const MarchGeom &m, int &col, int &ch) {
-  const int L = blockIdx.x;
+__device__ __forceinline__ bool march_tile_at(const MarchGeom &m, int L, int &col, int &ch) {
   const int tile = (L & 7) * m.per_xcd + (L >> 3);
   if (tile >= m.ncol * m.nchunk) return false;
   col = tile % m.ncol;
   ch = tile / m.ncol;
   return true;
+}
+
+__device__ __forceinline__ bool march_tile(const MarchGeom &m, int &col, int &ch) {
+  return march_tile_at(m, blockIdx.x, col, ch);
+}
+
+// Bit e set when cell k0 + e of a W-cell vector lies in a z-face slab of the phase
+// (MAG: H slabs, else E slabs): the cells a CPML_SPLIT_Z bulk kernel leaves to the
+// z-slab workgroups.
+template <typename T, int W, bool MAG>
+__device__ __forceinline__ unsigned zslab_bits(const FaceSet<T> &fs, int k0) {
+  unsigned b = 0;
+#pragma unroll
+  for (int e = 0; e < W; ++e)
+    if (in_slab<T, 2, MAG>(fs, k0 + e)) b |= 1u << e;
+  return b;
 }
 
 // The field and coefficient arrays as no-alias pointers: the kernel bodies below take
@@ -287,19 +302,18 @@ struct Fields {
 // plane before plane 0 and after plane nx, so all neighbour loads are unconditional;
 // elements outside a component's Yee shape are written back unchanged.
 template <typename T, int CM, int W, int FC>
-__device__ __forceinline__ void update_h_body(
+__device__ __forceinline__ void update_h_cells(
     T *__restrict__ Hx, T *__restrict__ Hy, T *__restrict__ Hz,
     const T *__restrict__ Ex, const T *__restrict__ Ey, const T *__restrict__ Ez,
-    const Grid<T> &g, const MarchGeom &m, const TableCache<T, CM> &tc, int col, int ch, const FaceSet<T> &fs) {
+    const Grid<T> &g, const TableCache<T, CM> &tc, int j, int k0, int i0, int chunk, const FaceSet<T> &fs) {
 #pragma clang fp contract(off)
   typedef Vec<T, W> V;
-  const int p = (col * 256 + (int)threadIdx.x) * W;
   const int nx = g.nx, ny = g.ny, nz = g.nz, SJ = g.SJ, SI = g.SI;
-  const int j = p / SJ;
-  const int k0 = p - j * SJ;
-  if (j > ny) return;                                      // plane padding
-  const int i0 = ch * m.chunk;
-  const int i1 = min(i0 + m.chunk, nx + 1);
+  const int p = j * SJ + k0;
+  const int i1 = min(i0 + chunk, nx + 1);
+  // CPML_SPLIT_Z: cells of a z slab belong to the z-slab workgroups.
+  const unsigned skip = FC == CPML_SPLIT_Z ? zslab_bits<T, W, true>(fs, k0) : 0u;
+  if (skip == (1u << W) - 1) return;
   int c = i0 * SI + p;
   V ey_i = ldv<T, W>(Ey + c), ez_i = ldv<T, W>(Ez + c);   // E at plane i, carried forward
   for (int i = i0; i < i1; ++i, c += SI) {
@@ -340,7 +354,7 @@ __device__ __forceinline__ void update_h_body(
         hz[e] = h;
       }
     }
-    if (FC != CPML_SEPARATE) {
+    if (FC == CPML_FUSE_Z || FC == CPML_FUSE_ALL) {
       // Cells the bulk kernel owns get all their CPML terms here, right after their
       // bulk update and in the reference's face order; k_cpml skips them.
 #pragma unroll
@@ -358,11 +372,33 @@ __device__ __forceinline__ void update_h_body(
               [&] { return dx_hz[e]; }, [&] { return dy_hz[e]; });
       }
     }
-    stv_stream<T, W>(Hx + c, hx);
-    if (inner) { stv_stream<T, W>(Hy + c, hy); stv_stream<T, W>(Hz + c, hz); }
+    if (skip) {
+#pragma unroll
+      for (int e = 0; e < W; ++e) {
+        if (skip >> e & 1u) continue;
+        Hx[c + e] = hx[e];
+        if (inner) { Hy[c + e] = hy[e]; Hz[c + e] = hz[e]; }
+      }
+    } else {
+      stv_stream<T, W>(Hx + c, hx);
+      if (inner) { stv_stream<T, W>(Hy + c, hy); stv_stream<T, W>(Hz + c, hz); }
+    }
     ey_i = ey_n;
     ez_i = ez_n;
   }
+}
+
+// The stencil tile (col, ch) of the H bulk update: W consecutive cells of a row per
+// thread, over the tile's x planes.
+template <typename T, int CM, int W, int FC>
+__device__ __forceinline__ void update_h_body(
+    T *__restrict__ Hx, T *__restrict__ Hy, T *__restrict__ Hz,
+    const T *__restrict__ Ex, const T *__restrict__ Ey, const T *__restrict__ Ez,
+    const Grid<T> &g, const MarchGeom &m, const TableCache<T, CM> &tc, int col, int ch, const FaceSet<T> &fs) {
+  const int p = (col * 256 + (int)threadIdx.x) * W;
+  const int j = p / g.SJ;
+  if (j > g.ny) return;                                    // plane padding
+  update_h_cells<T, CM, W, FC>(Hx, Hy, Hz, Ex, Ey, Ez, g, tc, j, p - j * g.SJ, ch * m.chunk, m.chunk, fs);
 }
 
 template <typename T, int CM, int W, int FC>
@@ -383,20 +419,19 @@ __global__ void __launch_bounds__(256) k_update_h(Grid<T> g, MarchGeom m, FaceSe
 // every face fused (FC == CPML_FUSE_ALL) the same thread then applies the wall's CPML
 // terms, which read only H.  Elements the bulk does not update keep their value.
 template <typename T, int CM, int W, int FC>
-__device__ __forceinline__ void update_e_body(
+__device__ __forceinline__ void update_e_cells(
     T *__restrict__ Ex, T *__restrict__ Ey, T *__restrict__ Ez,
     const T *__restrict__ Hx, const T *__restrict__ Hy, const T *__restrict__ Hz,
-    const Grid<T> &g, const MarchGeom &m, const TableCache<T, CM> &tc, int col, int ch, int ghost,
+    const Grid<T> &g, const TableCache<T, CM> &tc, int j, int k0, int i0, int chunk, int ghost,
     const FaceSet<T> &fs) {
 #pragma clang fp contract(off)
   typedef Vec<T, W> V;
-  const int p = (col * 256 + (int)threadIdx.x) * W;
   const int nx = g.nx, ny = g.ny, nz = g.nz, SJ = g.SJ, SI = g.SI;
-  const int j = p / SJ;
-  const int k0 = p - j * SJ;
-  if (j > ny) return;                                      // plane padding
-  const int i0 = ch * m.chunk;
-  const int i1 = min(i0 + m.chunk, nx);
+  const int p = j * SJ + k0;
+  const int i1 = min(i0 + chunk, nx);
+  // CPML_SPLIT_Z: cells of a z slab (wall cells included) belong to the z-slab workgroups.
+  const unsigned skip = FC == CPML_SPLIT_Z ? zslab_bits<T, W, false>(fs, k0) : 0u;
+  if (skip == (1u << W) - 1) return;
   const bool jin = j >= 1 && j < ny;
   int c = i0 * SI + p;
   V hz_m = ldv<T, W>(Hz + c - SI), hy_m = ldv<T, W>(Hy + c - SI);   // H at plane i-1
@@ -446,7 +481,7 @@ __device__ __forceinline__ void update_e_body(
       }
     }
     if (i == 0) ex0 = ex;
-    if (FC != CPML_SEPARATE) {
+    if (FC == CPML_FUSE_Z || FC == CPML_FUSE_ALL) {
       // As in update_h_body; ex0 keeps the pre-CPML Ex that the ghost plane reads.
 #pragma unroll
       for (int e = 0; e < W; ++e) {
@@ -463,9 +498,19 @@ __device__ __forceinline__ void update_e_body(
               [&] { return dx_ez[e]; }, [&] { return dy_ez[e]; });
       }
     }
-    if (i < nx) stv_stream<T, W>(Ex + c, ex);
-    stv_stream<T, W>(Ey + c, ey);
-    stv_stream<T, W>(Ez + c, ez);
+    if (skip) {
+#pragma unroll
+      for (int e = 0; e < W; ++e) {
+        if (skip >> e & 1u) continue;
+        if (i < nx) Ex[c + e] = ex[e];
+        Ey[c + e] = ey[e];
+        Ez[c + e] = ez[e];
+      }
+    } else {
+      if (i < nx) stv_stream<T, W>(Ex + c, ex);
+      stv_stream<T, W>(Ey + c, ey);
+      stv_stream<T, W>(Ez + c, ez);
+    }
 
     hz_m = hz_i;
     hy_m = hy_i;
@@ -478,6 +523,7 @@ __device__ __forceinline__ void update_e_body(
     const int cg = nx * SI + p;
 #pragma unroll
     for (int e = 0; e < W; ++e) {
+      if (skip >> e & 1u) continue;
       const int k = k0 + e;
       const bool kin = k >= 1 && k < nz;
       if (j < ny && kin) {                               // Ey(nx, j, k)
@@ -512,6 +558,19 @@ __device__ __forceinline__ void update_e_body(
   }
 }
 
+// The stencil tile (col, ch) of the E bulk update (see update_h_body).
+template <typename T, int CM, int W, int FC>
+__device__ __forceinline__ void update_e_body(
+    T *__restrict__ Ex, T *__restrict__ Ey, T *__restrict__ Ez,
+    const T *__restrict__ Hx, const T *__restrict__ Hy, const T *__restrict__ Hz,
+    const Grid<T> &g, const MarchGeom &m, const TableCache<T, CM> &tc, int col, int ch, int ghost,
+    const FaceSet<T> &fs) {
+  const int p = (col * 256 + (int)threadIdx.x) * W;
+  const int j = p / g.SJ;
+  if (j > g.ny) return;                                    // plane padding
+  update_e_cells<T, CM, W, FC>(Ex, Ey, Ez, Hx, Hy, Hz, g, tc, j, p - j * g.SJ, ch * m.chunk, m.chunk, ghost, fs);
+}
+
 template <typename T, int CM, int W, int FC>
 __global__ void __launch_bounds__(256) k_update_e(Grid<T> g, MarchGeom m, int ghost, FaceSet<T> fs) {
   __shared__ TableCache<T, CM> tc;
@@ -519,6 +578,48 @@ __global__ void __launch_bounds__(256) k_update_e(Grid<T> g, MarchGeom m, int gh
   if (!march_tile(m, col, ch)) return;
   load_tables<T, CM>(g, 3, tc);
   update_e_body<T, CM, W, FC>(g.Ex, g.Ey, g.Ez, g.Hx, g.Hy, g.Hz, g, m, tc, col, ch, ghost, fs);
+}
+
+// CPML_SPLIT_Z: one launch of the bulk update in which workgroups [0, z.ntiles) march
+// the z-slab cells (bulk update, then every CPML term, one cell per lane; the E wall's
+// bulk update for those rows with the plane-0 value of Ex they computed) and the rest
+// are the stencil tiles, which leave the z-slab cells alone.  The two sets write
+// disjoint cells and read only the other field, so their order does not matter.
+template <typename T, int CM, int W>
+__global__ void __launch_bounds__(256) k_update_h_split(Grid<T> g, MarchGeom m, ZslabGeom z, FaceSet<T> fs) {
+  __shared__ TableCache<T, CM> tc;
+  const int L = blockIdx.x;
+  int col, ch;
+  if (L < z.ntiles) {
+    if (!zslab_tile(z, L, col, ch)) return;
+    load_tables<T, CM>(g, 0, tc);
+    int j, k;
+    if (!zslab_lane(z, col * 256 + (int)threadIdx.x, j, k)) return;
+    update_h_cells<T, CM, 1, CPML_FUSE_Z>(g.Hx, g.Hy, g.Hz, g.Ex, g.Ey, g.Ez, g, tc, j, k, ch * z.chunk, z.chunk, fs);
+  } else {
+    if (!march_tile_at(m, L - z.ntiles, col, ch)) return;
+    load_tables<T, CM>(g, 0, tc);
+    update_h_body<T, CM, W, CPML_SPLIT_Z>(g.Hx, g.Hy, g.Hz, g.Ex, g.Ey, g.Ez, g, m, tc, col, ch, fs);
+  }
+}
+
+template <typename T, int CM, int W>
+__global__ void __launch_bounds__(256) k_update_e_split(Grid<T> g, MarchGeom m, ZslabGeom z, int ghost, FaceSet<T> fs) {
+  __shared__ TableCache<T, CM> tc;
+  const int L = blockIdx.x;
+  int col, ch;
+  if (L < z.ntiles) {
+    if (!zslab_tile(z, L, col, ch)) return;
+    load_tables<T, CM>(g, 3, tc);
+    int j, k;
+    if (!zslab_lane(z, col * 256 + (int)threadIdx.x, j, k)) return;
+    update_e_cells<T, CM, 1, CPML_FUSE_Z>(g.Ex, g.Ey, g.Ez, g.Hx, g.Hy, g.Hz, g, tc, j, k, ch * z.chunk, z.chunk,
+                                          ghost, fs);
+  } else {
+    if (!march_tile_at(m, L - z.ntiles, col, ch)) return;
+    load_tables<T, CM>(g, 3, tc);
+    update_e_body<T, CM, W, CPML_SPLIT_Z>(g.Ex, g.Ey, g.Ez, g.Hx, g.Hy, g.Hz, g, m, tc, col, ch, ghost, fs);
+  }
 }
 
 // ------------------------------------------------------------------ CPML pass --

@@ -2,7 +2,8 @@
 // g++ by test_cpml_tiles.py.  For each layout, phase and CPML placement it decodes
 // every lane of every tile exactly as the kernels do and checks, against a
 // brute-force enumeration of the grid, that each cell the pass owns is visited once,
-// with the mask of exactly the faces whose slab holds it, and nothing else.
+// with the mask of exactly the faces whose slab holds it, and nothing else; likewise
+// the z-slab march lanes of the split placement (check_zslab).
 //
 //   cpml_tiles_check            prints one line per case, then OK (exit 0) or FAIL
 #include "../special-topics_amd/csrc/cpml_tiles.hpp"
@@ -77,6 +78,52 @@ static int check(const CpmlLayout &L, const char *name) {
           }
       std::printf("%s mode %d phase %d: %zu tiles, %ld cells, lane use %.3f\n", name, mode, ph, tiles.size(), cells,
                   tiles.empty() ? 1.0 : (double)lanes / (256.0 * tiles.size()));
+    }
+  return bad;
+}
+
+// CPML_SPLIT_Z: the z-slab march workgroups of each phase visit every (i, j, k) with k
+// in a z slab of the phase, j in 0..ny and i in their planes (H: 0..nx, E: 0..nx-1)
+// exactly once, for several x chunk depths; together with the bulk kernels' cells
+// outside the z slabs and the FUSE_Z tile list this is FUSE_Z's decomposition.
+static int check_zslab(const CpmlLayout &L, const char *name) {
+  int bad = 0;
+  const int nx = L.n[0], ny = L.n[1], nz = L.n[2];
+  for (int ph = 0; ph < 2; ++ph)
+    for (int chunk : {1, 2, 3, 8}) {
+      const int planes = ph == 0 ? nx + 1 : nx;
+      const ZslabGeom z = zslab_geom(L, ph, planes, chunk);
+      std::vector<unsigned char> seen((size_t)(nx + 1) * (ny + 1) * (nz + 1), 0);
+      if (z.ntiles % 8) { std::printf("  %s: %d z-slab tiles, not a multiple of 8\n", name, z.ntiles); ++bad; }
+      for (int Lw = 0; Lw < z.ntiles; ++Lw) {
+        int col = 0, ch = 0;
+        if (!zslab_tile(z, Lw, col, ch)) continue;
+        for (int lane = 0; lane < 256; ++lane) {
+          int j = 0, k = 0;
+          if (!zslab_lane(z, col * 256 + lane, j, k)) continue;
+          for (int i = ch * z.chunk; i < std::min(ch * z.chunk + z.chunk, planes); ++i) {
+            if (j < 0 || j > ny || k < 0 || k > nz) {
+              if (bad++ < 5) std::printf("  %s zslab phase %d: lane outside (%d,%d,%d)\n", name, ph, i, j, k);
+              continue;
+            }
+            if (seen[((size_t)i * (ny + 1) + j) * (nz + 1) + k]++ && bad++ < 5)
+              std::printf("  %s zslab phase %d: (%d,%d,%d) twice\n", name, ph, i, j, k);
+          }
+        }
+      }
+      long cells = 0;
+      for (int i = 0; i <= nx; ++i)
+        for (int j = 0; j <= ny; ++j)
+          for (int k = 0; k <= nz; ++k) {
+            const int co[3] = {i, j, k};
+            const bool want = (faces_of(L, ph, co) & (3u << 4)) && i < planes;
+            cells += want;
+            if (want != (seen[((size_t)i * (ny + 1) + j) * (nz + 1) + k] != 0) && bad++ < 5)
+              std::printf("  %s zslab phase %d chunk %d: (%d,%d,%d) want=%d\n", name, ph, chunk, i, j, k, want);
+          }
+      if (chunk == 2)
+        std::printf("%s zslab phase %d: %d tiles, %ld cells, lane use %.3f\n", name, ph, z.ntiles, cells,
+                    (double)z.lanes * planes / (256.0 * z.ncol * planes));
     }
   return bad;
 }
@@ -183,7 +230,13 @@ int main() {
     char name[64];
     std::snprintf(name, sizeof name, "random %d (%dx%dx%d)", r, L.n[0], L.n[1], L.n[2]);
     bad += check(L, name);
+    bad += check_zslab(L, name);
   }
+  bad += check_zslab(standard(64, 72, 80, 8, 0), "64x72x80 npml 8");
+  bad += check_zslab(standard(39, 136, 136, 8, 0), "39x136x136 npml 8");
+  bad += check_zslab(standard(256, 256, 256, 8, 0), "256^3 npml 8");
+  bad += check_zslab(standard(21, 17, 70, 4, 1u << 5), "21x17x70 npml 4 no zp");
+  bad += check_zslab(standard(12, 12, 12, 0, 63u), "12^3 no CPML");
   std::printf(bad ? "FAIL (%d)\n" : "OK\n", bad);
   return bad != 0;
 }
