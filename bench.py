@@ -261,8 +261,9 @@ EARLYSTOP_WEIGHTS = os.path.join(REPO, "special-topics_amd", "fdtdhip", "earlyst
 def early_stop_study(count=4, batch_count=64, first=4600):
     """Config 5 time-to-solution: the full-length FP64 run to S11 against the FP32 run
     stopped at step 1500 + the network (HIP kernels) + S11, with the FP64 fallback when
-    the network's residual estimate exceeds 1e-4; once with `batch_count` variants in
-    one batched launch on each side, once for `count` variants one at a time."""
+    the network's S11-error estimate exceeds its calibrated threshold; once with
+    `batch_count` variants in one batched launch on each side (fallbacks re-run as their
+    own FP64 batch, timed), once for `count` variants one at a time."""
     from fdtdhip import earlystop
     if not os.path.exists(EARLYSTOP_WEIGHTS):
         return {"skipped": f"no trained weights at {EARLYSTOP_WEIGHTS}"}
@@ -270,6 +271,7 @@ def early_stop_study(count=4, batch_count=64, first=4600):
                        "early stop at step 1500; variants from 4600 on are outside the training and held-out ranges"}
     progress(f"early stop: {batch_count} variants batched on each side")
     out["batched"] = earlystop.time_to_solution_batched(EARLYSTOP_WEIGHTS, first, batch_count)
+    out["batched"].pop("per_variant", None)
     progress(f"early stop: {count} variants one at a time")
     out["single"] = earlystop.time_to_solution(EARLYSTOP_WEIGHTS, first, count)
     return out

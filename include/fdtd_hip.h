@@ -254,7 +254,10 @@ typedef struct {
     int fuse_cpml;             /* 0: all CPML terms in the separate CPML pass; 1: cells in
                                   the z-face slabs get all their CPML terms inside the
                                   bulk kernels, right after their bulk update; 2: every
-                                  CPML cell does, and there is no separate pass */
+                                  CPML cell does, and there is no separate pass; 3: as 1,
+                                  but the z-slab cells are updated (bulk and CPML) by
+                                  workgroups of their own in the bulk kernels' launch
+                                  (single problems only, not fdtd_hip_multi_*) */
 } FdtdHipTuning;
 
 int fdtd_hip_set_tuning(FdtdHipContext *ctx, const FdtdHipTuning *tuning);
@@ -312,12 +315,14 @@ int fdtd_hip_multi_autotune(FdtdHipMulti *m, int steps_per_candidate, FdtdHipTun
 void fdtd_hip_multi_destroy(FdtdHipMulti *m);
 
 /* Early-stop network (no counterpart in the reference): predicts the port-voltage and
- * port-current tail of an IFA run from its first 1500 steps (architecture and data
- * layout in special-topics_amd/fdtdhip/earlystop_net.py).  Weights are BF16 bit
- * patterns in layer order conv1..conv4, fc1, fc2 (each [out][in][k] / [out][in]),
- * biases FP32 in the same order; x is [batch][2][375] FP32, y [batch][1375] FP32. */
+ * port-current tail of an IFA run from its first 1500 steps, with two estimates of its
+ * own error (architecture and data layout in special-topics_amd/fdtdhip/earlystop_net.py).
+ * Weights are BF16 bit patterns in layer order conv1..conv4, fc1, fc2 (each
+ * [out][in][k] / [out][in]), biases FP32 in the same order; x is [batch][2][375] FP32,
+ * y [batch][1376] FP32: 2 x 687 pooled tail samples, log10 of the residual tail energy
+ * over the total, log10 of the maximum S11 error in dB. */
 #define FDTD_HIP_ES_INPUTS 750
-#define FDTD_HIP_ES_OUTPUTS 1375
+#define FDTD_HIP_ES_OUTPUTS 1376
 typedef struct FdtdHipEarlyStop FdtdHipEarlyStop;
 int fdtd_hip_es_create(const unsigned short *weights, long long nweights, const float *biases, long long nbiases,
                        FdtdHipEarlyStop **out);

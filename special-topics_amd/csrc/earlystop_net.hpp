@@ -1,6 +1,6 @@
 // Inference of the early-stop network (architecture in fdtdhip/earlystop_net.py):
 // four Conv1d(k=5, stride=2, pad=2) + ReLU layers of 64 channels over a [2][375]
-// input, then Linear(1536 -> 512) + ReLU and Linear(512 -> 1375).  Weights are BF16
+// input, then Linear(1536 -> 512) + ReLU and Linear(512 -> 1376).  Weights are BF16
 // bit patterns, activations and accumulation FP32.
 #pragma once
 
@@ -8,7 +8,7 @@
 
 namespace fdtd {
 
-constexpr int ES_IN_CH = 2, ES_CH = 64, ES_KW = 5, ES_HIDDEN = 512, ES_OUT = 1375;
+constexpr int ES_IN_CH = 2, ES_CH = 64, ES_KW = 5, ES_HIDDEN = 512, ES_OUT = 1376;
 constexpr int ES_LEN[5] = {375, 188, 94, 47, 24};
 
 __device__ __forceinline__ float bf16f(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
