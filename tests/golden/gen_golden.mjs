@@ -313,7 +313,8 @@ async function main() {
     console.error('steps golden-small');
   }
   if (what === 'run' || what === 'all') {
-    const plan = { 'golden-small': 160, 'ifa-matlab-7000': 7000, 'ifa-config1-7000': 7000 };
+    const plan = { 'golden-small': 160, 'ifa-matlab-7000': 7000, 'ifa-config1-7000': 7000,
+                   'ifa-fr4-substrate': 300, 'ifa-finefreq-sweep': 300 };
     for (const n of names.length ? names : Object.keys(plan)) {
       const P = refBuild(R, scen(n));
       const rec = runRecord(R, P, plan[n] || P.N);

@@ -149,9 +149,11 @@ def test_batches_compose(get_problem):
 @pytest.mark.parametrize("scenario,fuse", [("ifa-matlab-7000", 0), ("ifa-matlab-7000", 2),
                                            ("ifa-matlab-7000", 3), ("ifa-config1-7000", 0),
                                            ("ifa-config1-7000", 1), ("ifa-config1-7000", 2),
-                                           ("ifa-config1-7000", 3)])
+                                           ("ifa-config1-7000", 3), ("ifa-fr4-substrate", 0),
+                                           ("ifa-finefreq-sweep", 0)])
 def test_ifa_fp64_full_run_matches_reference(get_problem, scenario, fuse):
-    """The 39x136x136 IFA, 7000 FP64 steps, against the reference's own per-step code."""
+    """The 39x136x136 IFA scenarios (7000 FP64 steps; 300 for the FR4-substrate and
+    fine-frequency ones) against the reference's own per-step code."""
     prob = get_problem(scenario)
     fx = G.load(f"run-{scenario}.json")
     sim = _sim(prob)

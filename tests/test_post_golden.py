@@ -25,6 +25,8 @@ SCEN = {
     "golden-small": os.path.join(REPO, "tests", "golden", "scenarios", "golden-small.json"),
     "ifa-matlab-7000": os.path.join(REPO, "special-topics_amd", "scenarios", "ifa-matlab-7000.json"),
     "ifa-config1-7000": os.path.join(REPO, "special-topics_amd", "scenarios", "ifa-config1-7000.json"),
+    "ifa-fr4-substrate": os.path.join(REPO, "special-topics_amd", "scenarios", "ifa-fr4-substrate.json"),
+    "ifa-finefreq-sweep": os.path.join(REPO, "special-topics_amd", "scenarios", "ifa-finefreq-sweep.json"),
 }
 
 
@@ -48,7 +50,8 @@ def same(b64_a: str, b64_b: str) -> bool:
     return G.f64(b64_a).tobytes() == G.f64(b64_b).tobytes()
 
 
-@pytest.mark.parametrize("scenario", ["golden-small", "ifa-matlab-7000", "ifa-config1-7000"])
+@pytest.mark.parametrize("scenario", ["golden-small", "ifa-matlab-7000", "ifa-config1-7000", "ifa-fr4-substrate",
+                                      "ifa-finefreq-sweep"])
 def test_port_dft_and_s11_bit_exact(tmp_path, scenario):
     fx = G.load(f"run-{scenario}.json")
     out = run_post(tmp_path, scenario, fx["steps"], fx["v"], fx["i"])

@@ -9,7 +9,7 @@ import pytest
 
 import goldens as G
 
-SCENARIOS = ["ifa-matlab-7000", "ifa-config1-7000", "golden-small"]
+SCENARIOS = ["ifa-matlab-7000", "ifa-config1-7000", "ifa-fr4-substrate", "ifa-finefreq-sweep", "golden-small"]
 
 
 @pytest.mark.parametrize("scenario", SCENARIOS)
