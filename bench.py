@@ -235,12 +235,13 @@ def sweep_study(workdir, count=64, batch=64, seed=4):
     sweep = Sweep("fp32", "matlab", keep_traces=False, batch=batch)
     res, secs, cells = sweep.run(files)
     dims = np.array([r["dims"] for r in res])
+    span = lambda a: f"{a.min()}-{a.max()}"
     res_ghz = np.array([r["resonance_hz"] for r in res]) / 1e9
     smin = np.array([r["s11_min_db"] for r in res])
     dpk = np.array([r["peak_directivity_dbi"] for r in res])
     return {
         "workload": f"{count} seeded IFA variants (gen_ifa_variants.py seed {seed}), grids "
-                    f"{dims[:, 0].min()}x{dims[:, 1].min()}-{dims[:, 1].max()}x{dims[:, 2].min()}-{dims[:, 2].max()}, "
+                    f"{span(dims[:, 0])}x{span(dims[:, 1])}x{span(dims[:, 2])}, "
                     "FP32, 7000 steps, matlab boundary, S11 at 1000 f, NTFF at 16 f (matrix-core blocked)",
         "batch": batch, "batch_tuning": list(sweep.batch_tuning), "build_s": round(build_s, 2),
         "seconds": round(secs, 3),
