@@ -336,7 +336,7 @@ def stencil_study(workload, precision, steps, warmup, timing_steps, workdir, ntf
     nh, ne = psi_counts(problem)
     cb = cell_phase_bytes(esz, coef_mode)
     dom = 0 if avg_ms[0] >= avg_ms[1] else 1
-    fused_axes = {0: "", 1: "z", 2: "xyz", 3: "z"}[tuning[3]]
+    fused_axes = {0: "", 1: "z", 2: "xyz", 3: "z", 4: "xyz"}[tuning[3]]
     fh, fe = psi_counts(problem, fused_axes) if fused_axes else (0, 0)
     psi_fused = PSI_VALUES * esz * (fh if dom == 0 else fe)
     dom_bytes = cb * ncells + psi_fused
@@ -347,6 +347,8 @@ def stencil_study(workload, precision, steps, warmup, timing_steps, workdir, ntf
     ctype = "float" if fp32 else "double"
     def stencil_name(phase):
         w = max(tuning[1], 1)
+        if tuning[3] == 4:          # every slab cell in workgroups of their own
+            return f"fdtd::k_update_{phase}_all<{ctype}, {coef_mode}, {w}>"
         if tuning[3] == 3:          # z slabs in workgroups of their own: the split kernels
             return f"fdtd::k_update_{phase}_split<{ctype}, {coef_mode}, {w}>"
         return f"fdtd::k_update_{phase}<{ctype}, {coef_mode}, {w}, {tuning[3]}>"
@@ -357,7 +359,7 @@ def stencil_study(workload, precision, steps, warmup, timing_steps, workdir, ntf
     # Whole step, counted: the two stencils, the two CPML passes and the small E-tail
     # kernels as the timing steps launch them (separately).
     step_kernels = [stencil_name("h"), stencil_name("e")]
-    if tuning[3] != 2:
+    if tuning[3] not in (2, 4):
         step_kernels += [f"fdtd::k_cpml_h<{ctype}>", f"fdtd::k_cpml_e<{ctype}>"]
     step_traffic = sum(table[k] for k in step_kernels) if all(k in table for k in step_kernels) else None
     step_bytes = 2 * cb * ncells + PSI_VALUES * esz * (nh + ne)
@@ -397,7 +399,8 @@ def stencil_study(workload, precision, steps, warmup, timing_steps, workdir, ntf
         "tuning": {"planes_per_workgroup": tuning[0], "cells_per_thread": tuning[1],
                    "coefficient_storage": ["indexed", "arrays"][tuning[2]],
                    "cpml": ["separate pass", "z slabs fused into bulk", "all faces fused into bulk",
-                            "z slabs in their own workgroups of the bulk launch"][tuning[3]]},
+                            "z slabs in their own workgroups of the bulk launch",
+                            "every slab cell in workgroups of its own in the bulk launch"][tuning[3]]},
     }
 
 

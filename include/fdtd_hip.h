@@ -256,8 +256,10 @@ typedef struct {
                                   bulk kernels, right after their bulk update; 2: every
                                   CPML cell does, and there is no separate pass; 3: as 1,
                                   but the z-slab cells are updated (bulk and CPML) by
-                                  workgroups of their own in the bulk kernels' launch
-                                  (single problems only, not fdtd_hip_multi_*) */
+                                  workgroups of their own in the bulk kernels' launch;
+                                  4: every CPML cell is, so each phase is one launch
+                                  and there is no separate pass (3 and 4: single
+                                  problems only, not fdtd_hip_multi_*) */
 } FdtdHipTuning;
 
 int fdtd_hip_set_tuning(FdtdHipContext *ctx, const FdtdHipTuning *tuning);

@@ -26,10 +26,19 @@ namespace fdtd {
 // the z-face slab cells (bulk update and every CPML term) in workgroups of their own
 // within the bulk kernels' launch -- FUSE_Z's split of the work, without z-slab cells
 // in the bulk kernels' waves.  SPLIT_Z leaves the CPML pass the same cells as FUSE_Z.
-constexpr int CPML_SEPARATE = 0, CPML_FUSE_Z = 1, CPML_FUSE_ALL = 2, CPML_SPLIT_Z = 3;
+// SPLIT_ALL makes each phase one launch: the stencil tiles leave every slab cell alone,
+// and the CPML tiles of the separate pass's list, in workgroups of the same launch, give
+// each of their cells its bulk update and then its CPML terms (the E wall plane goes
+// with the cell of plane 0 at the same (j, k)).
+constexpr int CPML_SEPARATE = 0, CPML_FUSE_Z = 1, CPML_FUSE_ALL = 2, CPML_SPLIT_Z = 3, CPML_SPLIT_ALL = 4;
 
 // The tile list (build_cpml_tiles mode) the CPML pass uses under a placement.
-constexpr int cpml_tile_mode(int placement) { return placement == CPML_SPLIT_Z ? CPML_FUSE_Z : placement; }
+constexpr int cpml_tile_mode(int placement) {
+  return placement == CPML_SPLIT_Z ? CPML_FUSE_Z : (placement == CPML_SPLIT_ALL ? CPML_SEPARATE : placement);
+}
+
+// Whether a separate CPML pass runs after the bulk kernels under a placement.
+constexpr bool cpml_pass_separate(int placement) { return placement != CPML_SPLIT_ALL; }
 
 struct CpmlTile {
   unsigned x, y, z, w;

@@ -528,9 +528,45 @@ void launch_e_split(FdtdHipContext *c, const Grid<T> &g, const MarchGeom &m, int
   else hipLaunchKernelGGL((k_update_e_split<T, CM, 1>), grid, block, 0, c->stream, g, m, z, ghost, fs);
 }
 
+// CPML_SPLIT_ALL launches: the CPML tiles (padded to a multiple of 8 workgroups), then
+// the stencil tiles.
+template <typename T, int CM>
+void launch_h_all(FdtdHipContext *c, const Grid<T> &g, const MarchGeom &m, const FaceSet<T> &fs) {
+  const int nt = (int)c->ncpml[0];
+  const dim3 grid(((nt + 7) & ~7) + 8 * m.per_xcd), block(256);
+  const CpmlTile *t = c->cpml_tiles[0];
+  const int w = cells_per_thread(c);
+  if constexpr (sizeof(T) == 4) {
+    if (w == 4) { hipLaunchKernelGGL((k_update_h_all<T, CM, 4>), grid, block, 0, c->stream, g, m, fs, t, nt); return; }
+  }
+  if (w == 2) hipLaunchKernelGGL((k_update_h_all<T, CM, 2>), grid, block, 0, c->stream, g, m, fs, t, nt);
+  else hipLaunchKernelGGL((k_update_h_all<T, CM, 1>), grid, block, 0, c->stream, g, m, fs, t, nt);
+}
+
+template <typename T, int CM>
+void launch_e_all(FdtdHipContext *c, const Grid<T> &g, const MarchGeom &m, int ghost, const FaceSet<T> &fs) {
+  const int nt = (int)c->ncpml[1];
+  const dim3 grid(((nt + 7) & ~7) + 8 * m.per_xcd), block(256);
+  const CpmlTile *t = c->cpml_tiles[1];
+  const int w = cells_per_thread(c);
+  if constexpr (sizeof(T) == 4) {
+    if (w == 4) {
+      hipLaunchKernelGGL((k_update_e_all<T, CM, 4>), grid, block, 0, c->stream, g, m, ghost, fs, t, nt);
+      return;
+    }
+  }
+  if (w == 2) hipLaunchKernelGGL((k_update_e_all<T, CM, 2>), grid, block, 0, c->stream, g, m, ghost, fs, t, nt);
+  else hipLaunchKernelGGL((k_update_e_all<T, CM, 1>), grid, block, 0, c->stream, g, m, ghost, fs, t, nt);
+}
+
 template <typename T>
 void launch_h(FdtdHipContext *c, const Grid<T> &g, const MarchGeom &m, const FaceSet<T> &fs) {
   switch (fuse_cpml(c)) {
+    case CPML_SPLIT_ALL:
+      if (c->coef_mode == COEF_U8) launch_h_all<T, COEF_U8>(c, g, m, fs);
+      else if (c->coef_mode == COEF_U16) launch_h_all<T, COEF_U16>(c, g, m, fs);
+      else launch_h_all<T, COEF_ARRAYS>(c, g, m, fs);
+      break;
     case CPML_FUSE_ALL: launch_h_f<T, CPML_FUSE_ALL>(c, g, m, fs); break;
     case CPML_FUSE_Z: launch_h_f<T, CPML_FUSE_Z>(c, g, m, fs); break;
     case CPML_SPLIT_Z:
@@ -545,6 +581,11 @@ void launch_h(FdtdHipContext *c, const Grid<T> &g, const MarchGeom &m, const Fac
 template <typename T>
 void launch_e(FdtdHipContext *c, const Grid<T> &g, const MarchGeom &m, int ghost, const FaceSet<T> &fs) {
   switch (fuse_cpml(c)) {
+    case CPML_SPLIT_ALL:
+      if (c->coef_mode == COEF_U8) launch_e_all<T, COEF_U8>(c, g, m, ghost, fs);
+      else if (c->coef_mode == COEF_U16) launch_e_all<T, COEF_U16>(c, g, m, ghost, fs);
+      else launch_e_all<T, COEF_ARRAYS>(c, g, m, ghost, fs);
+      break;
     case CPML_FUSE_ALL: launch_e_f<T, CPML_FUSE_ALL>(c, g, m, ghost, fs); break;
     case CPML_FUSE_Z: launch_e_f<T, CPML_FUSE_Z>(c, g, m, ghost, fs); break;
     case CPML_SPLIT_Z:
@@ -664,12 +705,15 @@ int launch_step(FdtdHipContext *c, int ts, const int *tsp = nullptr) {
   const int ghost = c->d.boundary == FDTD_HIP_BOUNDARY_REFERENCE;
   const MarchGeom mh = march_geom(c, c->d.nx + 1), me = march_geom(c, ghost ? c->d.nx : c->d.nx + 1);
   const bool t = c->timing;
+  // CPML tiles of the separate pass (none when the bulk launches hold them).
+  const bool pass = cpml_pass_separate(fuse_cpml(c));
+  const long pass_h = pass ? c->ncpml[0] : 0, pass_e = pass ? c->ncpml[1] : 0;
 
   if (t) HIP_TRY(hipEventRecord(c->ev[0], c->stream));
   launch_h<T>(c, g, mh, fs);
   if (t) HIP_TRY(hipEventRecord(c->ev[1], c->stream));
-  if (c->ncpml[0])
-    hipLaunchKernelGGL(k_cpml_h<T>, dim3((unsigned)c->ncpml[0]), block, 0, c->stream, g, fs, c->cpml_tiles[0]);
+  if (pass_h)
+    hipLaunchKernelGGL(k_cpml_h<T>, dim3((unsigned)pass_h), block, 0, c->stream, g, fs, c->cpml_tiles[0]);
   if (t) HIP_TRY(hipEventRecord(c->ev[2], c->stream));
   launch_e<T>(c, g, me, ghost, fs);
   if (t) HIP_TRY(hipEventRecord(c->ev[3], c->stream));
@@ -695,7 +739,7 @@ int launch_step(FdtdHipContext *c, int ts, const int *tsp = nullptr) {
   c->e_tail_merged = !t && c->e_tail_ok;
   if (c->e_tail_merged) {
     const int nf = (int)((total + 255) / 256), pp = ports ? 1 : 0;
-    const long nblk = c->ncpml[1] + pp + nf;
+    const long nblk = pass_e + pp + nf;
     if (nblk > 0) {
       const dim3 grid((unsigned)nblk);
       if (nkind == 2)
@@ -706,8 +750,8 @@ int launch_step(FdtdHipContext *c, int ts, const int *tsp = nullptr) {
         hipLaunchKernelGGL((k_e_tail<T, 0>), grid, block, 0, c->stream, g, fs, c->cpml_tiles[1], pp, nf, ps, c->ff, nb, row, tsp, ts);
     }
   } else {
-    if (c->ncpml[1])
-      hipLaunchKernelGGL(k_cpml_e<T>, dim3((unsigned)c->ncpml[1]), block, 0, c->stream, g, fs, c->cpml_tiles[1]);
+    if (pass_e)
+      hipLaunchKernelGGL(k_cpml_e<T>, dim3((unsigned)pass_e), block, 0, c->stream, g, fs, c->cpml_tiles[1]);
     if (t) HIP_TRY(hipEventRecord(c->ev[4], c->stream));
     if (ports) hipLaunchKernelGGL(k_port_epilogue<T>, dim3(1), block, 0, c->stream, g, ps, tsp, ts);
     if (t) HIP_TRY(hipEventRecord(c->ev[5], c->stream));
@@ -1397,8 +1441,8 @@ static int check_tuning(const FdtdHipContext *c, const FdtdHipTuning *t, const c
     return fail("%s: %d cells per thread unsupported (1, 2%s)", who, t->cells_per_thread, maxw == 4 ? " or 4" : "");
   if (t->coefficient_storage < 0 || t->coefficient_storage > 1)
     return fail("%s: coefficient storage %d unknown", who, t->coefficient_storage);
-  if (t->fuse_cpml < CPML_SEPARATE || t->fuse_cpml > CPML_SPLIT_Z)
-    return fail("%s: fuse_cpml %d unknown (0, 1, 2 or 3)", who, t->fuse_cpml);
+  if (t->fuse_cpml < CPML_SEPARATE || t->fuse_cpml > CPML_SPLIT_ALL)
+    return fail("%s: fuse_cpml %d unknown (0 to 4)", who, t->fuse_cpml);
   return FDTD_HIP_OK;
 }
 
@@ -1459,7 +1503,7 @@ int fdtd_hip_autotune(FdtdHipContext *c, int steps_per_candidate, FdtdHipTuning 
     for (int w : ws) {
       if (w == 4 && !c->fp32()) continue;
       for (int pl : planes)
-        for (int fc = CPML_SEPARATE; fc <= CPML_SPLIT_Z; ++fc) cands.push_back(FdtdHipTuning{pl, w, cs, fc});
+        for (int fc = CPML_SEPARATE; fc <= CPML_SPLIT_ALL; ++fc) cands.push_back(FdtdHipTuning{pl, w, cs, fc});
     }
   }
   double best = 1e300;
@@ -1960,7 +2004,8 @@ int fdtd_hip_multi_create(FdtdHipContext *const *ctxs, int n, const FdtdHipTunin
   if (tuning) {
     t = *tuning;
     if (check_tuning(ctxs[0], &t, "multi_create")) return FDTD_HIP_ERR;
-    if (t.fuse_cpml == CPML_SPLIT_Z) return fail("multi_create: CPML placement 3 (split z slabs) is single-problem only");
+    if (t.fuse_cpml == CPML_SPLIT_Z || t.fuse_cpml == CPML_SPLIT_ALL)
+      return fail("multi_create: CPML placements 3 and 4 (split slabs) are single-problem only");
   }
   HIP_TRY(hipSetDevice(ctxs[0]->device));
   FdtdHipMulti *m = new FdtdHipMulti();

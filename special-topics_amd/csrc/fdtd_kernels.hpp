@@ -244,6 +244,87 @@ __device__ __forceinline__ bool fused_cpml_cell(const FaceSet<T> &fs, int i, int
   return false;
 }
 
+template <typename T>
+struct CpmlDiff {
+  const T *P;
+  int hi, lo;
+};
+
+template <typename T>
+struct CpmlStage {
+  T v, u1, w1, u2, w2;
+  T ps[4], cp[4];
+  T b[4], a[4];     // staged 1-D profile values (FP32 only; see cpml_stage_profile)
+};
+
+// Whether the 1-D profile values b and a are staged with the other loads or read at
+// store time.  They come from arrays of nc entries that stay in cache.  In FP32 the
+// kernel's SGPRs set its occupancy, so staging them costs nothing and saves a round
+// trip (measured: reading them at store time made the FP32 pass 5% slower).  In FP64
+// staging them holds 16 more VGPRs per component, and reading them at store time takes
+// the pass from 133 VGPRs (3 waves per SIMD) to 89 (5 waves), 6% faster.
+template <typename T>
+constexpr bool cpml_stage_profile() { return sizeof(T) == 4; }
+
+template <typename T, int A, int SLOT, bool MAG>
+__device__ __forceinline__ void cpml_load_face(const FaceSet<T> &fs, int s, int i, int j, int k, int d1, int d2,
+                                               T &ps, T &cp, T &b, T &a) {
+  const Face<T> &F = fs.f[2 * A + s];
+  const int ci = (A == 0 ? i : (A == 1 ? j : k)) - (MAG ? F.m_start : F.e_start);
+  const int q = psi_offset<A>(i, j, k, ci, F.nc, d1, d2);
+  ps = F.psi[SLOT][q];
+  cp = F.cpsi[SLOT][q];
+  if (cpml_stage_profile<T>()) {
+    b = MAG ? F.b_m[ci] : F.b_e[ci];
+    a = MAG ? F.a_m[ci] : F.a_e[ci];
+  }
+}
+
+template <typename T, int A, int SLOT, bool MAG>
+__device__ __forceinline__ T cpml_store_face(const FaceSet<T> &fs, int s, int i, int j, int k, int d1, int d2,
+                                             T v, T diff, T ps, T cp, T b, T a) {
+  const Face<T> &F = fs.f[2 * A + s];
+  const int ci = (A == 0 ? i : (A == 1 ? j : k)) - (MAG ? F.m_start : F.e_start);
+  if (!cpml_stage_profile<T>()) {
+    b = MAG ? F.b_m[ci] : F.b_e[ci];
+    a = MAG ? F.a_m[ci] : F.a_e[ci];
+  }
+  const T p = b * ps + a * diff;
+  F.psi[SLOT][psi_offset<A>(i, j, k, ci, F.nc, d1, d2)] = p;
+  return v + cp * p;
+}
+
+// Psi, CPsi (and in FP32 the profile values) of the faces in `mask` of one component
+// whose terms come from the faces of axes A1 and A2.
+template <typename T, int A1, int S1, int A2, int S2, bool MAG>
+__device__ __forceinline__ void cpml_stage_faces(const FaceSet<T> &fs, unsigned mask, int i, int j, int k, int d1,
+                                                 int d2, CpmlStage<T> &L) {
+  if (mask & (1u << (2 * A1))) cpml_load_face<T, A1, S1, MAG>(fs, 0, i, j, k, d1, d2, L.ps[0], L.cp[0], L.b[0], L.a[0]);
+  if (mask & (2u << (2 * A1))) cpml_load_face<T, A1, S1, MAG>(fs, 1, i, j, k, d1, d2, L.ps[1], L.cp[1], L.b[1], L.a[1]);
+  if (mask & (1u << (2 * A2))) cpml_load_face<T, A2, S2, MAG>(fs, 0, i, j, k, d1, d2, L.ps[2], L.cp[2], L.b[2], L.a[2]);
+  if (mask & (2u << (2 * A2))) cpml_load_face<T, A2, S2, MAG>(fs, 1, i, j, k, d1, d2, L.ps[3], L.cp[3], L.b[3], L.a[3]);
+}
+
+// The staged faces' Psi recurrences (curl differences e1 along A1, e2 along A2) and
+// field corrections applied to v, in the reference's order; returns the field value.
+template <typename T, int A1, int S1, int A2, int S2, bool MAG>
+__device__ __forceinline__ T cpml_apply_faces(const FaceSet<T> &fs, unsigned mask, int i, int j, int k, int d1, int d2,
+                                              T v, T e1, T e2, const CpmlStage<T> &L) {
+#pragma clang fp contract(off)
+  if (mask & (1u << (2 * A1)))
+    v = cpml_store_face<T, A1, S1, MAG>(fs, 0, i, j, k, d1, d2, v, e1, L.ps[0], L.cp[0], L.b[0], L.a[0]);
+  if (mask & (2u << (2 * A1)))
+    v = cpml_store_face<T, A1, S1, MAG>(fs, 1, i, j, k, d1, d2, v, e1, L.ps[1], L.cp[1], L.b[1], L.a[1]);
+  if (mask & (1u << (2 * A2)))
+    v = cpml_store_face<T, A2, S2, MAG>(fs, 0, i, j, k, d1, d2, v, e2, L.ps[2], L.cp[2], L.b[2], L.a[2]);
+  if (mask & (2u << (2 * A2)))
+    v = cpml_store_face<T, A2, S2, MAG>(fs, 1, i, j, k, d1, d2, v, e2, L.ps[3], L.cp[3], L.b[3], L.a[3]);
+  return v;
+}
+
+// Face bits of each axis in a tile mask.
+constexpr unsigned CPML_X = 3u, CPML_Y = 3u << 2, CPML_Z = 3u << 4;
+
 // Launch geometry shared by the two stencil kernels.  A thread updates W consecutive
 // cells of one row (rows are padded to a multiple of 4 elements, so the W cells never
 // straddle a row); a workgroup's 256 threads cover 256*W consecutive plane offsets
@@ -285,6 +366,23 @@ __device__ __forceinline__ unsigned zslab_bits(const FaceSet<T> &fs, int k0) {
   return b;
 }
 
+// The cells of row j (W cells from k0) that a stencil tile of placement FC leaves to
+// other workgroups, before the x-plane is known: CPML_SPLIT_Z leaves z-slab cells,
+// CPML_SPLIT_ALL every cell of a y or z slab (x slabs: split_plane_bits).
+template <typename T, int W, int FC, bool MAG>
+__device__ __forceinline__ unsigned split_row_bits(const FaceSet<T> &fs, int j, int k0) {
+  if (FC == CPML_SPLIT_ALL && in_slab<T, 1, MAG>(fs, j)) return (1u << W) - 1;
+  if (FC == CPML_SPLIT_Z || FC == CPML_SPLIT_ALL) return zslab_bits<T, W, MAG>(fs, k0);
+  return 0u;
+}
+
+// The same for plane i: CPML_SPLIT_ALL also leaves every cell of an x-slab plane.
+template <typename T, int W, int FC, bool MAG>
+__device__ __forceinline__ unsigned split_plane_bits(const FaceSet<T> &fs, unsigned row_bits, int i) {
+  if (FC == CPML_SPLIT_ALL && in_slab<T, 0, MAG>(fs, i)) return (1u << W) - 1;
+  return row_bits;
+}
+
 // The field and coefficient arrays as no-alias pointers: the kernel bodies below take
 // them as __restrict__ parameters so the compiler may issue the next plane's loads
 // ahead of this plane's stores.
@@ -311,14 +409,22 @@ __device__ __forceinline__ void update_h_cells(
   const int nx = g.nx, ny = g.ny, nz = g.nz, SJ = g.SJ, SI = g.SI;
   const int p = j * SJ + k0;
   const int i1 = min(i0 + chunk, nx + 1);
-  // CPML_SPLIT_Z: cells of a z slab belong to the z-slab workgroups.
-  const unsigned skip = FC == CPML_SPLIT_Z ? zslab_bits<T, W, true>(fs, k0) : 0u;
-  if (skip == (1u << W) - 1) return;
+  constexpr unsigned ALL = (1u << W) - 1;
+  // CPML_SPLIT_Z / CPML_SPLIT_ALL: cells of the slabs these placements hand to other
+  // workgroups of the launch are neither updated nor stored here.
+  const unsigned row_skip = split_row_bits<T, W, FC, true>(fs, j, k0);
+  if (row_skip == ALL) return;
   int c = i0 * SI + p;
   V ey_i = ldv<T, W>(Ey + c), ez_i = ldv<T, W>(Ez + c);   // E at plane i, carried forward
   for (int i = i0; i < i1; ++i, c += SI) {
     const bool inner = i < nx;
+    const unsigned skip = split_plane_bits<T, W, FC, true>(fs, row_skip, i);
     const V ey_n = ldv<T, W>(Ey + c + SI), ez_n = ldv<T, W>(Ez + c + SI);
+    if (skip == ALL) {
+      ey_i = ey_n;
+      ez_i = ez_n;
+      continue;
+    }
     const V ex_i = ldv<T, W>(Ex + c);
     const V ez_j = ldv<T, W>(Ez + c + SJ), ex_j = ldv<T, W>(Ex + c + SJ);
     const T ey_w = Ey[c + W], ex_w = Ex[c + W];           // first cell past the vector
@@ -410,6 +516,51 @@ __global__ void __launch_bounds__(256) k_update_h(Grid<T> g, MarchGeom m, FaceSe
   update_h_body<T, CM, W, FC>(g.Hx, g.Hy, g.Hz, g.Ex, g.Ey, g.Ez, g, m, tc, col, ch, fs);
 }
 
+// Bulk update of the i = nx wall's Ey/Ez at (j, k) under the reference boundary (see
+// update_e_cells); ex0 is Ex(0, j, k) after this step's bulk update, before its CPML
+// terms.
+template <typename T>
+__device__ __forceinline__ void e_wall_bulk(const Grid<T> &g, T *__restrict__ Ey, T *__restrict__ Ez,
+                                            const T *__restrict__ Hx, const T *__restrict__ Hy,
+                                            const T *__restrict__ Hz, int j, int k, T ex0) {
+#pragma clang fp contract(off)
+  const int nx = g.nx, ny = g.ny, nz = g.nz, SJ = g.SJ, SI = g.SI;
+  const int cg = nx * SI + j * SJ + k;
+  const bool jin = j >= 1 && j < ny, kin = k >= 1 && k < nz;
+  if (j < ny && kin) {                                   // Ey(nx, j, k)
+    const T dHx = Hx[cg] - Hx[cg - 1];
+    const T dHz = ex0 - Hz[cg - SI];
+    T v = g.C[CEYE][cg] * Ey[cg] + g.C[CEYHX][cg] * dHx + g.C[CEYHZ][cg] * dHz;
+    Ey[cg] = v;
+  }
+  if (jin && k < nz) {                                   // Ez(nx, j, k)
+    const int f = j * nz + k;                            // reference flat index into Hz
+    const int fr = f / (nz + 1);
+    const T dHy = Hz[fr * SJ + (f - fr * (nz + 1))] - Hy[cg - SI];
+    const T dHx = Hx[cg] - Hx[cg - SJ];
+    T v = g.C[CEZE][cg] * Ez[cg] + g.C[CEZHY][cg] * dHy + g.C[CEZHX][cg] * dHx;
+    Ez[cg] = v;
+  }
+}
+
+// CPML terms of the i = nx wall's Ey/Ez at (j, k), after its bulk update (or none,
+// under the matlab boundary), in the reference's face order.
+template <typename T>
+__device__ __forceinline__ void e_wall_terms(const Grid<T> &g, const FaceSet<T> &fs, T *__restrict__ Ey,
+                                             T *__restrict__ Ez, const T *__restrict__ Hx, const T *__restrict__ Hy,
+                                             const T *__restrict__ Hz, int j, int k) {
+#pragma clang fp contract(off)
+  const int nx = g.nx, ny = g.ny, nz = g.nz, SJ = g.SJ, SI = g.SI;
+  const int ce = nx * SI + j * SJ + k;
+  if (k > nz || !fused_cpml_cell<T, CPML_FUSE_ALL, false>(fs, nx, j, k)) return;
+  if (j < ny)                                            // Ey(nx): x then z faces
+    Ey[ce] = cpml_terms<T, 0, 2, 2, 3, false>(fs, Ey[ce], nx, j, k, ny, nz + 1,
+        [&] { return Hz[ce] - Hz[ce - SI]; }, [&] { return Hx[ce] - Hx[ce - 1]; });
+  if (k < nz)                                            // Ez(nx): x then y faces
+    Ez[ce] = cpml_terms<T, 0, 3, 1, 2, false>(fs, Ez[ce], nx, j, k, ny + 1, nz,
+        [&] { return Hy[ce] - Hy[ce - SI]; }, [&] { return Hx[ce] - Hx[ce - SJ]; });
+}
+
 // E bulk update; W consecutive cells of a row per thread.  With ghost != 0 (reference
 // boundary) the E planes are i in [0,nx) and the thread at i = 0 also updates Ey/Ez on
 // the i = nx wall the way the reference's updateE does: Ey there differences
@@ -429,16 +580,26 @@ __device__ __forceinline__ void update_e_cells(
   const int nx = g.nx, ny = g.ny, nz = g.nz, SJ = g.SJ, SI = g.SI;
   const int p = j * SJ + k0;
   const int i1 = min(i0 + chunk, nx);
-  // CPML_SPLIT_Z: cells of a z slab (wall cells included) belong to the z-slab workgroups.
-  const unsigned skip = FC == CPML_SPLIT_Z ? zslab_bits<T, W, false>(fs, k0) : 0u;
-  if (skip == (1u << W) - 1) return;
+  constexpr unsigned ALL = (1u << W) - 1;
+  // CPML_SPLIT_Z / CPML_SPLIT_ALL: cells (wall cells included) of the slabs these
+  // placements hand to other workgroups of the launch are left alone here.
+  const unsigned row_skip = split_row_bits<T, W, FC, false>(fs, j, k0);
+  if (row_skip == ALL) return;
+  // The wall cells of this row go with the cells of plane 0.
+  const unsigned skip0 = split_plane_bits<T, W, FC, false>(fs, row_skip, 0);
   const bool jin = j >= 1 && j < ny;
   int c = i0 * SI + p;
   V hz_m = ldv<T, W>(Hz + c - SI), hy_m = ldv<T, W>(Hy + c - SI);   // H at plane i-1
   V ex0 = V{};     // Ex at plane 0 after its bulk update, before any CPML term (the wall reads it)
   for (int i = i0; i < i1; ++i, c += SI) {
     const bool iin = i >= 1 && i < nx;
+    const unsigned skip = split_plane_bits<T, W, FC, false>(fs, row_skip, i);
     const V hz_i = ldv<T, W>(Hz + c), hy_i = ldv<T, W>(Hy + c), hx_i = ldv<T, W>(Hx + c);
+    if (skip == ALL) {
+      hz_m = hz_i;
+      hy_m = hy_i;
+      continue;
+    }
     const V hz_j = ldv<T, W>(Hz + c - SJ), hx_j = ldv<T, W>(Hx + c - SJ);
     const T hy_w = Hy[c - 1], hx_w = Hx[c - 1];            // last cell before the vector
     V ex = ldv_stream<T, W>(Ex + c), ey = ldv_stream<T, W>(Ey + c), ez = ldv_stream<T, W>(Ez + c);
@@ -518,42 +679,15 @@ __device__ __forceinline__ void update_e_cells(
   // The i = nx wall, once per column by the chunk that holds plane 0.  Only this thread
   // writes the wall's Ey/Ez in this phase, and H does not change, so the wall may follow
   // the march; ex0 is Ex(0) as the bulk left it.  Kept out of the march loop, its
-  // registers do not add to the loop's.
-  if (ghost && i0 == 0) {
-    const int cg = nx * SI + p;
+  // registers do not add to the loop's.  With CPML_FUSE_ALL / CPML_SPLIT_ALL the same
+  // thread then applies the wall's CPML terms.
+  constexpr bool WALL_TERMS = FC == CPML_FUSE_ALL || FC == CPML_SPLIT_ALL;
+  if ((ghost || WALL_TERMS) && i0 == 0) {
 #pragma unroll
     for (int e = 0; e < W; ++e) {
-      if (skip >> e & 1u) continue;
-      const int k = k0 + e;
-      const bool kin = k >= 1 && k < nz;
-      if (j < ny && kin) {                               // Ey(nx, j, k)
-        const T dHx = Hx[cg + e] - Hx[cg + e - 1];
-        const T dHz = ex0[e] - Hz[cg + e - SI];
-        T v = g.C[CEYE][cg + e] * Ey[cg + e] + g.C[CEYHX][cg + e] * dHx + g.C[CEYHZ][cg + e] * dHz;
-        Ey[cg + e] = v;
-      }
-      if (jin && k < nz) {                               // Ez(nx, j, k)
-        const int f = j * nz + k;                        // reference flat index into Hz
-        const int fr = f / (nz + 1);
-        const T dHy = Hz[fr * SJ + (f - fr * (nz + 1))] - Hy[cg + e - SI];
-        const T dHx = Hx[cg + e] - Hx[cg + e - SJ];
-        T v = g.C[CEZE][cg + e] * Ez[cg + e] + g.C[CEZHY][cg + e] * dHy + g.C[CEZHX][cg + e] * dHx;
-        Ez[cg + e] = v;
-      }
-    }
-  }
-  if (FC == CPML_FUSE_ALL && i0 == 0) {
-    const int cg = nx * SI + p;
-#pragma unroll
-    for (int e = 0; e < W; ++e) {
-      const int k = k0 + e, ce = cg + e;
-      if (k > nz || !fused_cpml_cell<T, FC, false>(fs, nx, j, k)) continue;
-      if (j < ny)                                        // Ey(nx): x then z faces
-        Ey[ce] = cpml_terms<T, 0, 2, 2, 3, false>(fs, Ey[ce], nx, j, k, ny, nz + 1,
-            [&] { return Hz[ce] - Hz[ce - SI]; }, [&] { return Hx[ce] - Hx[ce - 1]; });
-      if (k < nz)                                        // Ez(nx): x then y faces
-        Ez[ce] = cpml_terms<T, 0, 3, 1, 2, false>(fs, Ez[ce], nx, j, k, ny + 1, nz,
-            [&] { return Hy[ce] - Hy[ce - SI]; }, [&] { return Hx[ce] - Hx[ce - SJ]; });
+      if (skip0 >> e & 1u) continue;
+      if (ghost) e_wall_bulk<T>(g, Ey, Ez, Hx, Hy, Hz, j, k0 + e, ex0[e]);
+      if (WALL_TERMS) e_wall_terms<T>(g, fs, Ey, Ez, Hx, Hy, Hz, j, k0 + e);
     }
   }
 }
@@ -632,56 +766,6 @@ __global__ void __launch_bounds__(256) k_update_e_split(Grid<T> g, MarchGeom m, 
 // then p, then of A2; cpml.js:470,484) and the stores.  Issuing the loads of all
 // three components before any arithmetic or store keeps them in flight together.
 // The curl difference along an axis is P[c + hi] - P[c + lo].
-template <typename T>
-struct CpmlDiff {
-  const T *P;
-  int hi, lo;
-};
-
-template <typename T>
-struct CpmlStage {
-  T v, u1, w1, u2, w2;
-  T ps[4], cp[4];
-  T b[4], a[4];     // staged 1-D profile values (FP32 only; see cpml_stage_profile)
-};
-
-// Whether the 1-D profile values b and a are staged with the other loads or read at
-// store time.  They come from arrays of nc entries that stay in cache.  In FP32 the
-// kernel's SGPRs set its occupancy, so staging them costs nothing and saves a round
-// trip (measured: reading them at store time made the FP32 pass 5% slower).  In FP64
-// staging them holds 16 more VGPRs per component, and reading them at store time takes
-// the pass from 133 VGPRs (3 waves per SIMD) to 89 (5 waves), 6% faster.
-template <typename T>
-constexpr bool cpml_stage_profile() { return sizeof(T) == 4; }
-
-template <typename T, int A, int SLOT, bool MAG>
-__device__ __forceinline__ void cpml_load_face(const FaceSet<T> &fs, int s, int i, int j, int k, int d1, int d2,
-                                               T &ps, T &cp, T &b, T &a) {
-  const Face<T> &F = fs.f[2 * A + s];
-  const int ci = (A == 0 ? i : (A == 1 ? j : k)) - (MAG ? F.m_start : F.e_start);
-  const int q = psi_offset<A>(i, j, k, ci, F.nc, d1, d2);
-  ps = F.psi[SLOT][q];
-  cp = F.cpsi[SLOT][q];
-  if (cpml_stage_profile<T>()) {
-    b = MAG ? F.b_m[ci] : F.b_e[ci];
-    a = MAG ? F.a_m[ci] : F.a_e[ci];
-  }
-}
-
-template <typename T, int A, int SLOT, bool MAG>
-__device__ __forceinline__ T cpml_store_face(const FaceSet<T> &fs, int s, int i, int j, int k, int d1, int d2,
-                                             T v, T diff, T ps, T cp, T b, T a) {
-  const Face<T> &F = fs.f[2 * A + s];
-  const int ci = (A == 0 ? i : (A == 1 ? j : k)) - (MAG ? F.m_start : F.e_start);
-  if (!cpml_stage_profile<T>()) {
-    b = MAG ? F.b_m[ci] : F.b_e[ci];
-    a = MAG ? F.a_m[ci] : F.a_e[ci];
-  }
-  const T p = b * ps + a * diff;
-  F.psi[SLOT][psi_offset<A>(i, j, k, ci, F.nc, d1, d2)] = p;
-  return v + cp * p;
-}
-
 template <typename T, int A1, int S1, int A2, int S2, bool MAG>
 __device__ __forceinline__ void cpml_stage_load(const FaceSet<T> &fs, unsigned mask, const T *F, int c, int i, int j,
                                                 int k, int d1, int d2, CpmlDiff<T> x1, CpmlDiff<T> x2,
@@ -689,31 +773,16 @@ __device__ __forceinline__ void cpml_stage_load(const FaceSet<T> &fs, unsigned m
   L.v = F[c];
   if (mask & (3u << (2 * A1))) { L.u1 = x1.P[c + x1.hi]; L.w1 = x1.P[c + x1.lo]; }
   if (mask & (3u << (2 * A2))) { L.u2 = x2.P[c + x2.hi]; L.w2 = x2.P[c + x2.lo]; }
-  if (mask & (1u << (2 * A1))) cpml_load_face<T, A1, S1, MAG>(fs, 0, i, j, k, d1, d2, L.ps[0], L.cp[0], L.b[0], L.a[0]);
-  if (mask & (2u << (2 * A1))) cpml_load_face<T, A1, S1, MAG>(fs, 1, i, j, k, d1, d2, L.ps[1], L.cp[1], L.b[1], L.a[1]);
-  if (mask & (1u << (2 * A2))) cpml_load_face<T, A2, S2, MAG>(fs, 0, i, j, k, d1, d2, L.ps[2], L.cp[2], L.b[2], L.a[2]);
-  if (mask & (2u << (2 * A2))) cpml_load_face<T, A2, S2, MAG>(fs, 1, i, j, k, d1, d2, L.ps[3], L.cp[3], L.b[3], L.a[3]);
+  cpml_stage_faces<T, A1, S1, A2, S2, MAG>(fs, mask, i, j, k, d1, d2, L);
 }
 
 template <typename T, int A1, int S1, int A2, int S2, bool MAG>
 __device__ __forceinline__ void cpml_stage_store(const FaceSet<T> &fs, unsigned mask, T *F, int c, int i, int j,
                                                  int k, int d1, int d2, const CpmlStage<T> &L) {
 #pragma clang fp contract(off)
-  T v = L.v;
-  const T e1 = L.u1 - L.w1, e2 = L.u2 - L.w2;
-  if (mask & (1u << (2 * A1)))
-    v = cpml_store_face<T, A1, S1, MAG>(fs, 0, i, j, k, d1, d2, v, e1, L.ps[0], L.cp[0], L.b[0], L.a[0]);
-  if (mask & (2u << (2 * A1)))
-    v = cpml_store_face<T, A1, S1, MAG>(fs, 1, i, j, k, d1, d2, v, e1, L.ps[1], L.cp[1], L.b[1], L.a[1]);
-  if (mask & (1u << (2 * A2)))
-    v = cpml_store_face<T, A2, S2, MAG>(fs, 0, i, j, k, d1, d2, v, e2, L.ps[2], L.cp[2], L.b[2], L.a[2]);
-  if (mask & (2u << (2 * A2)))
-    v = cpml_store_face<T, A2, S2, MAG>(fs, 1, i, j, k, d1, d2, v, e2, L.ps[3], L.cp[3], L.b[3], L.a[3]);
-  F[c] = v;
+  F[c] = cpml_apply_faces<T, A1, S1, A2, S2, MAG>(fs, mask, i, j, k, d1, d2, L.v, L.u1 - L.w1, L.u2 - L.w2, L);
 }
 
-// Face bits of each axis in a tile mask.
-constexpr unsigned CPML_X = 3u, CPML_Y = 3u << 2, CPML_Z = 3u << 4;
 
 // Magnetic CPML terms of one tile's H elements, faces in the reference's order per
 // component (x, then y, then z; cpml.js:456-724).  Runs after the H bulk update, as
@@ -778,6 +847,145 @@ __global__ void __launch_bounds__(256) k_cpml_h(Grid<T> g, FaceSet<T> fs, const 
 template <typename T>
 __global__ void __launch_bounds__(256) k_cpml_e(Grid<T> g, FaceSet<T> fs, const CpmlTile *__restrict__ tiles) {
   cpml_e_tile<T>(g, fs, tiles[blockIdx.x], threadIdx.x);
+}
+
+// ------------------------------------------------- one launch per phase (SPLIT_ALL) --
+
+// H cell (i, j, k) of a CPML tile under CPML_SPLIT_ALL: its bulk update (the
+// expressions of update_h_cells), then the CPML terms of every face in the tile's
+// mask (those of cpml_h_tile), one cell per lane.  All loads are issued first.
+template <typename T, int CM>
+__device__ __forceinline__ void h_cell_split_all(const Grid<T> &g, const FaceSet<T> &fs, const TableCache<T, CM> &tc,
+                                                 unsigned mask, int i, int j, int k) {
+#pragma clang fp contract(off)
+  typedef Vec<T, 1> V;
+  const int nx = g.nx, ny = g.ny, nz = g.nz, SJ = g.SJ, SI = g.SI;
+  const int c = i * SI + j * SJ + k;
+  const T *__restrict__ Ex = g.Ex;
+  const T *__restrict__ Ey = g.Ey;
+  const T *__restrict__ Ez = g.Ez;
+  const bool inner = i < nx;
+  const bool ux = j < ny && k < nz, uy = inner && k < nz, uz = inner && j < ny;   // k <= nz in a tile
+  const bool px = ux && (mask & (CPML_Y | CPML_Z)), py = uy && (mask & (CPML_X | CPML_Z)),
+             pz = uz && (mask & (CPML_X | CPML_Y));
+  T hx = g.Hx[c], hy = g.Hy[c], hz = g.Hz[c];
+  const T ex = Ex[c], ey = Ey[c], ez = Ez[c];
+  const T ey_k = Ey[c + 1], ex_k = Ex[c + 1], ez_j = Ez[c + SJ], ex_j = Ex[c + SJ];
+  const T ez_n = Ez[c + SI], ey_n = Ey[c + SI];
+  V c0, c1, c2, c3, c4, c5, c6, c7, c8;
+  coeffs<T, CM, 1>(g.ct, tc, 0, 0, c, g.C[CHXH], g.C[CHXEY], g.C[CHXEZ], c0, c1, c2);
+  coeffs<T, CM, 1>(g.ct, tc, 0, 1, c, g.C[CHYH], g.C[CHYEZ], g.C[CHYEX], c3, c4, c5);
+  coeffs<T, CM, 1>(g.ct, tc, 0, 2, c, g.C[CHZH], g.C[CHZEX], g.C[CHZEY], c6, c7, c8);
+  CpmlStage<T> Lx, Ly, Lz;
+  if (px) cpml_stage_faces<T, 1, 0, 2, 0, true>(fs, mask, i, j, k, ny, nz, Lx);
+  if (py) cpml_stage_faces<T, 0, 0, 2, 1, true>(fs, mask, i, j, k, ny + 1, nz, Ly);
+  if (pz) cpml_stage_faces<T, 0, 1, 1, 1, true>(fs, mask, i, j, k, ny, nz + 1, Lz);
+  const T dz_hx = ey_k - ey, dy_hx = ez_j - ez;
+  const T dx_hy = ez_n - ez, dz_hy = ex_k - ex;
+  const T dy_hz = ex_j - ex, dx_hz = ey_n - ey;
+  if (ux) hx = c0[0] * hx + c1[0] * dz_hx + c2[0] * dy_hx;
+  if (uy) hy = c3[0] * hy + c4[0] * dx_hy + c5[0] * dz_hy;
+  if (uz) hz = c6[0] * hz + c7[0] * dy_hz + c8[0] * dx_hz;
+  if (px) hx = cpml_apply_faces<T, 1, 0, 2, 0, true>(fs, mask, i, j, k, ny, nz, hx, dy_hx, dz_hx, Lx);
+  if (py) hy = cpml_apply_faces<T, 0, 0, 2, 1, true>(fs, mask, i, j, k, ny + 1, nz, hy, dx_hy, dz_hy, Ly);
+  if (pz) hz = cpml_apply_faces<T, 0, 1, 1, 1, true>(fs, mask, i, j, k, ny, nz + 1, hz, dx_hz, dy_hz, Lz);
+  if (ux) g.Hx[c] = hx;
+  if (uy) g.Hy[c] = hy;
+  if (uz) g.Hz[c] = hz;
+}
+
+// E cell (i, j, k) of a CPML tile under CPML_SPLIT_ALL (update_e_cells' bulk
+// expressions, then cpml_e_tile's terms).  Cells of the i = nx wall are left to the
+// cell of plane 0 at the same (j, k), which needs Ex(0) before its CPML terms: that
+// lane does the wall's bulk update (reference boundary) and then its CPML terms.
+template <typename T, int CM>
+__device__ __forceinline__ void e_cell_split_all(const Grid<T> &g, const FaceSet<T> &fs, const TableCache<T, CM> &tc,
+                                                 unsigned mask, int i, int j, int k, int ghost) {
+#pragma clang fp contract(off)
+  typedef Vec<T, 1> V;
+  const int nx = g.nx, ny = g.ny, nz = g.nz, SJ = g.SJ, SI = g.SI;
+  if (i >= nx) return;
+  const int c = i * SI + j * SJ + k;
+  const T *__restrict__ Hx = g.Hx;
+  const T *__restrict__ Hy = g.Hy;
+  const T *__restrict__ Hz = g.Hz;
+  const bool iin = i >= 1, jin = j >= 1 && j < ny, kin = k >= 1 && k < nz;
+  const bool ux = jin && kin, uy = j < ny && iin && kin, uz = k < nz && iin && jin;
+  const bool px = mask & (CPML_Y | CPML_Z), py = j < ny && (mask & (CPML_X | CPML_Z)),
+             pz = k < nz && (mask & (CPML_X | CPML_Y));
+  T ex = g.Ex[c], ey = g.Ey[c], ez = g.Ez[c];
+  const T hx = Hx[c], hy = Hy[c], hz = Hz[c];
+  const T hz_j = Hz[c - SJ], hx_j = Hx[c - SJ], hy_k = Hy[c - 1], hx_k = Hx[c - 1];
+  const T hz_m = Hz[c - SI], hy_m = Hy[c - SI];
+  V c0, c1, c2, c3, c4, c5, c6, c7, c8;
+  coeffs<T, CM, 1>(g.ct, tc, 3, 0, c, g.C[CEXE], g.C[CEXHZ], g.C[CEXHY], c0, c1, c2);
+  coeffs<T, CM, 1>(g.ct, tc, 3, 1, c, g.C[CEYE], g.C[CEYHX], g.C[CEYHZ], c3, c4, c5);
+  coeffs<T, CM, 1>(g.ct, tc, 3, 2, c, g.C[CEZE], g.C[CEZHY], g.C[CEZHX], c6, c7, c8);
+  CpmlStage<T> Lx, Ly, Lz;
+  if (px) cpml_stage_faces<T, 1, 3, 2, 2, false>(fs, mask, i, j, k, ny + 1, nz + 1, Lx);
+  if (py) cpml_stage_faces<T, 0, 2, 2, 3, false>(fs, mask, i, j, k, ny, nz + 1, Ly);
+  if (pz) cpml_stage_faces<T, 0, 3, 1, 2, false>(fs, mask, i, j, k, ny + 1, nz, Lz);
+  const T dy_ex = hz - hz_j, dz_ex = hy - hy_k;
+  const T dz_ey = hx - hx_k, dx_ey = hz - hz_m;
+  const T dx_ez = hy - hy_m, dy_ez = hx - hx_j;
+  if (ux) ex = c0[0] * ex + c1[0] * dy_ex + c2[0] * dz_ex;
+  if (uy) ey = c3[0] * ey + c4[0] * dz_ey + c5[0] * dx_ey;
+  if (uz) ez = c6[0] * ez + c7[0] * dx_ez + c8[0] * dy_ez;
+  const T ex0 = ex;
+  if (px) ex = cpml_apply_faces<T, 1, 3, 2, 2, false>(fs, mask, i, j, k, ny + 1, nz + 1, ex, dy_ex, dz_ex, Lx);
+  if (py) ey = cpml_apply_faces<T, 0, 2, 2, 3, false>(fs, mask, i, j, k, ny, nz + 1, ey, dx_ey, dz_ey, Ly);
+  if (pz) ez = cpml_apply_faces<T, 0, 3, 1, 2, false>(fs, mask, i, j, k, ny + 1, nz, ez, dx_ez, dy_ez, Lz);
+  g.Ex[c] = ex;
+  if (j < ny) g.Ey[c] = ey;
+  if (k < nz) g.Ez[c] = ez;
+  if (i == 0) {
+    if (ghost) e_wall_bulk<T>(g, g.Ey, g.Ez, Hx, Hy, Hz, j, k, ex0);
+    e_wall_terms<T>(g, fs, g.Ey, g.Ez, Hx, Hy, Hz, j, k);
+  }
+}
+
+// CPML_SPLIT_ALL: one launch per phase.  Workgroups [0, ntiles) are the CPML tiles
+// (bulk update and CPML terms of every slab cell), padded to a multiple of 8 so the
+// stencil tiles after them keep their XCD placement; the stencil tiles leave every
+// slab cell alone.  The two sets write disjoint cells and read only the other field.
+template <typename T, int CM, int W>
+__global__ void __launch_bounds__(256) k_update_h_all(Grid<T> g, MarchGeom m, FaceSet<T> fs,
+                                                      const CpmlTile *__restrict__ tiles, int ntiles) {
+  __shared__ TableCache<T, CM> tc;
+  const int L = blockIdx.x, nt8 = (ntiles + 7) & ~7;
+  if (L < nt8) {
+    if (L >= ntiles) return;
+    load_tables<T, CM>(g, 0, tc);
+    unsigned mask;
+    int i, j, k;
+    if (!cpml_tile_cell(tiles[L], (int)threadIdx.x, mask, i, j, k)) return;
+    h_cell_split_all<T, CM>(g, fs, tc, mask, i, j, k);
+  } else {
+    int col, ch;
+    if (!march_tile_at(m, L - nt8, col, ch)) return;
+    load_tables<T, CM>(g, 0, tc);
+    update_h_body<T, CM, W, CPML_SPLIT_ALL>(g.Hx, g.Hy, g.Hz, g.Ex, g.Ey, g.Ez, g, m, tc, col, ch, fs);
+  }
+}
+
+template <typename T, int CM, int W>
+__global__ void __launch_bounds__(256) k_update_e_all(Grid<T> g, MarchGeom m, int ghost, FaceSet<T> fs,
+                                                      const CpmlTile *__restrict__ tiles, int ntiles) {
+  __shared__ TableCache<T, CM> tc;
+  const int L = blockIdx.x, nt8 = (ntiles + 7) & ~7;
+  if (L < nt8) {
+    if (L >= ntiles) return;
+    load_tables<T, CM>(g, 3, tc);
+    unsigned mask;
+    int i, j, k;
+    if (!cpml_tile_cell(tiles[L], (int)threadIdx.x, mask, i, j, k)) return;
+    e_cell_split_all<T, CM>(g, fs, tc, mask, i, j, k, ghost);
+  } else {
+    int col, ch;
+    if (!march_tile_at(m, L - nt8, col, ch)) return;
+    load_tables<T, CM>(g, 3, tc);
+    update_e_body<T, CM, W, CPML_SPLIT_ALL>(g.Ex, g.Ey, g.Ez, g.Hx, g.Hy, g.Hz, g, m, tc, col, ch, ghost, fs);
+  }
 }
 
 // ---------------------------------------------------------------- port epilogue --
