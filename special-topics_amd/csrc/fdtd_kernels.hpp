@@ -84,11 +84,15 @@ __device__ __forceinline__ Vec<T, W> ldv(const T *__restrict__ p) { return *(con
 template <typename T, int W>
 __device__ __forceinline__ void stv(T *__restrict__ p, const Vec<T, W> &v) { *(Vec<T, W> *)p = v; }
 
-// Non-temporal store of W cells (the own field of a stencil: written once, read
-// next by the other phase from HBM).
-template <typename T, int W>
+// Store of W cells.  NT (streaming) marks the own field of a stencil as written once
+// and read next by the other phase from HBM, so it does not evict the neighbour
+// planes; problems whose whole state fits in the on-die caches use plain stores
+// instead, so the next kernel finds the field there.
+template <typename T, int W, bool NT = true>
 __device__ __forceinline__ void stv_stream(T *__restrict__ p, const Vec<T, W> &v) {
-  if (W * sizeof(T) == 16) {
+  if (!NT) {
+    stv<T, W>(p, v);
+  } else if (W * sizeof(T) == 16) {
     typedef float f4 __attribute__((ext_vector_type(4)));
     f4 x;
     __builtin_memcpy(&x, &v, 16);
@@ -104,10 +108,12 @@ __device__ __forceinline__ void stv_stream(T *__restrict__ p, const Vec<T, W> &v
   }
 }
 
-template <typename T, int W>
+template <typename T, int W, bool NT = true>
 __device__ __forceinline__ Vec<T, W> ldv_stream(const T *__restrict__ p) {
   Vec<T, W> r;
-  if (W * sizeof(T) == 16) {
+  if (!NT) {
+    r = ldv<T, W>(p);
+  } else if (W * sizeof(T) == 16) {
     typedef float f4 __attribute__((ext_vector_type(4)));
     const f4 x = __builtin_nontemporal_load((const f4 *)p);
     __builtin_memcpy(&r, &x, 16);
@@ -124,35 +130,37 @@ __device__ __forceinline__ Vec<T, W> ldv_stream(const T *__restrict__ p) {
 
 // The (self, curl1, curl2) coefficients of component `first + local` for the W cells
 // starting at offset c.
-template <typename T, int CM, int W>
+template <typename T, int CM, int W, bool NT = true>
 __device__ __forceinline__ void coeffs(const CoefTables &ct, const TableCache<T, CM> &tc, int first, int local,
                                        int c, const T *__restrict__ a0, const T *__restrict__ a1,
                                        const T *__restrict__ a2, Vec<T, W> &c0, Vec<T, W> &c1, Vec<T, W> &c2) {
   if (CM == COEF_ARRAYS) {
-    c0 = ldv_stream<T, W>(a0 + c);
-    c1 = ldv_stream<T, W>(a1 + c);
-    c2 = ldv_stream<T, W>(a2 + c);
+    c0 = ldv_stream<T, W, NT>(a0 + c);
+    c1 = ldv_stream<T, W, NT>(a1 + c);
+    c2 = ldv_stream<T, W, NT>(a2 + c);
     return;
   }
+  // Index loads: streaming (NT) or plain, as the fields.
+  auto ld = [](auto *q) { return NT ? __builtin_nontemporal_load(q) : *q; };
   int id[W];
   if (CM == COEF_U8) {
     const uint8_t *ip = (const uint8_t *)ct.idx[first + local] + c;
     if (W == 4) {
-      const uint32_t x = __builtin_nontemporal_load((const uint32_t *)ip);
+      const uint32_t x = ld((const uint32_t *)ip);
 #pragma unroll
       for (int e = 0; e < W; ++e) id[e] = 3 * (int)((x >> (8 * e)) & 0xff);
     } else if (W == 2) {
-      const uint16_t x = __builtin_nontemporal_load((const uint16_t *)ip);
+      const uint16_t x = ld((const uint16_t *)ip);
 #pragma unroll
       for (int e = 0; e < W; ++e) id[e] = 3 * (int)((x >> (8 * e)) & 0xff);
     } else {
 #pragma unroll
-      for (int e = 0; e < W; ++e) id[e] = 3 * (int)__builtin_nontemporal_load(ip + e);
+      for (int e = 0; e < W; ++e) id[e] = 3 * (int)ld(ip + e);
     }
   } else {
     const uint16_t *ip = (const uint16_t *)ct.idx[first + local] + c;
 #pragma unroll
-    for (int e = 0; e < W; ++e) id[e] = 3 * (int)__builtin_nontemporal_load(ip + e);
+    for (int e = 0; e < W; ++e) id[e] = 3 * (int)ld(ip + e);
   }
   const T *t = CM == COEF_U8 ? tc.v + local * 3 * U8_TABLE : (const T *)ct.tab[first + local];
 #pragma unroll
@@ -399,7 +407,7 @@ struct Fields {
 // gives the same result as the separate k_cpml pass.  Every array has a zero guard
 // plane before plane 0 and after plane nx, so all neighbour loads are unconditional;
 // elements outside a component's Yee shape are written back unchanged.
-template <typename T, int CM, int W, int FC>
+template <typename T, int CM, int W, int FC, bool NT = true>
 __device__ __forceinline__ void update_h_cells(
     T *__restrict__ Hx, T *__restrict__ Hy, T *__restrict__ Hz,
     const T *__restrict__ Ex, const T *__restrict__ Ey, const T *__restrict__ Ez,
@@ -428,11 +436,11 @@ __device__ __forceinline__ void update_h_cells(
     const V ex_i = ldv<T, W>(Ex + c);
     const V ez_j = ldv<T, W>(Ez + c + SJ), ex_j = ldv<T, W>(Ex + c + SJ);
     const T ey_w = Ey[c + W], ex_w = Ex[c + W];           // first cell past the vector
-    V hx = ldv_stream<T, W>(Hx + c), hy = ldv_stream<T, W>(Hy + c), hz = ldv_stream<T, W>(Hz + c);
+    V hx = ldv_stream<T, W, NT>(Hx + c), hy = ldv_stream<T, W, NT>(Hy + c), hz = ldv_stream<T, W, NT>(Hz + c);
     V c0, c1, c2, c3, c4, c5, c6, c7, c8;
-    coeffs<T, CM, W>(g.ct, tc, 0, 0, c, g.C[CHXH], g.C[CHXEY], g.C[CHXEZ], c0, c1, c2);
-    coeffs<T, CM, W>(g.ct, tc, 0, 1, c, g.C[CHYH], g.C[CHYEZ], g.C[CHYEX], c3, c4, c5);
-    coeffs<T, CM, W>(g.ct, tc, 0, 2, c, g.C[CHZH], g.C[CHZEX], g.C[CHZEY], c6, c7, c8);
+    coeffs<T, CM, W, NT>(g.ct, tc, 0, 0, c, g.C[CHXH], g.C[CHXEY], g.C[CHXEZ], c0, c1, c2);
+    coeffs<T, CM, W, NT>(g.ct, tc, 0, 1, c, g.C[CHYH], g.C[CHYEZ], g.C[CHYEX], c3, c4, c5);
+    coeffs<T, CM, W, NT>(g.ct, tc, 0, 2, c, g.C[CHZH], g.C[CHZEX], g.C[CHZEY], c6, c7, c8);
     // The six E differences of the curls; the CPML terms use the same ones (E does not
     // change during the H phase).
     V dy_hx, dz_hx, dx_hy, dz_hy, dx_hz, dy_hz;
@@ -486,8 +494,8 @@ __device__ __forceinline__ void update_h_cells(
         if (inner) { Hy[c + e] = hy[e]; Hz[c + e] = hz[e]; }
       }
     } else {
-      stv_stream<T, W>(Hx + c, hx);
-      if (inner) { stv_stream<T, W>(Hy + c, hy); stv_stream<T, W>(Hz + c, hz); }
+      stv_stream<T, W, NT>(Hx + c, hx);
+      if (inner) { stv_stream<T, W, NT>(Hy + c, hy); stv_stream<T, W, NT>(Hz + c, hz); }
     }
     ey_i = ey_n;
     ez_i = ez_n;
@@ -496,7 +504,7 @@ __device__ __forceinline__ void update_h_cells(
 
 // The stencil tile (col, ch) of the H bulk update: W consecutive cells of a row per
 // thread, over the tile's x planes.
-template <typename T, int CM, int W, int FC>
+template <typename T, int CM, int W, int FC, bool NT = true>
 __device__ __forceinline__ void update_h_body(
     T *__restrict__ Hx, T *__restrict__ Hy, T *__restrict__ Hz,
     const T *__restrict__ Ex, const T *__restrict__ Ey, const T *__restrict__ Ez,
@@ -504,16 +512,16 @@ __device__ __forceinline__ void update_h_body(
   const int p = (col * 256 + (int)threadIdx.x) * W;
   const int j = p / g.SJ;
   if (j > g.ny) return;                                    // plane padding
-  update_h_cells<T, CM, W, FC>(Hx, Hy, Hz, Ex, Ey, Ez, g, tc, j, p - j * g.SJ, ch * m.chunk, m.chunk, fs);
+  update_h_cells<T, CM, W, FC, NT>(Hx, Hy, Hz, Ex, Ey, Ez, g, tc, j, p - j * g.SJ, ch * m.chunk, m.chunk, fs);
 }
 
-template <typename T, int CM, int W, int FC>
+template <typename T, int CM, int W, int FC, bool NT = true>
 __global__ void __launch_bounds__(256) k_update_h(Grid<T> g, MarchGeom m, FaceSet<T> fs) {
   __shared__ TableCache<T, CM> tc;
   int col, ch;
   if (!march_tile(m, col, ch)) return;
   load_tables<T, CM>(g, 0, tc);
-  update_h_body<T, CM, W, FC>(g.Hx, g.Hy, g.Hz, g.Ex, g.Ey, g.Ez, g, m, tc, col, ch, fs);
+  update_h_body<T, CM, W, FC, NT>(g.Hx, g.Hy, g.Hz, g.Ex, g.Ey, g.Ez, g, m, tc, col, ch, fs);
 }
 
 // Bulk update of the i = nx wall's Ey/Ez at (j, k) under the reference boundary (see
@@ -569,7 +577,7 @@ __device__ __forceinline__ void e_wall_terms(const Grid<T> &g, const FaceSet<T> 
 // element j*nz+k of Hz.  With ghost == 0 the wall is left to the CPML terms.  With
 // every face fused (FC == CPML_FUSE_ALL) the same thread then applies the wall's CPML
 // terms, which read only H.  Elements the bulk does not update keep their value.
-template <typename T, int CM, int W, int FC>
+template <typename T, int CM, int W, int FC, bool NT = true>
 __device__ __forceinline__ void update_e_cells(
     T *__restrict__ Ex, T *__restrict__ Ey, T *__restrict__ Ez,
     const T *__restrict__ Hx, const T *__restrict__ Hy, const T *__restrict__ Hz,
@@ -602,11 +610,11 @@ __device__ __forceinline__ void update_e_cells(
     }
     const V hz_j = ldv<T, W>(Hz + c - SJ), hx_j = ldv<T, W>(Hx + c - SJ);
     const T hy_w = Hy[c - 1], hx_w = Hx[c - 1];            // last cell before the vector
-    V ex = ldv_stream<T, W>(Ex + c), ey = ldv_stream<T, W>(Ey + c), ez = ldv_stream<T, W>(Ez + c);
+    V ex = ldv_stream<T, W, NT>(Ex + c), ey = ldv_stream<T, W, NT>(Ey + c), ez = ldv_stream<T, W, NT>(Ez + c);
     V c0, c1, c2, c3, c4, c5, c6, c7, c8;
-    coeffs<T, CM, W>(g.ct, tc, 3, 0, c, g.C[CEXE], g.C[CEXHZ], g.C[CEXHY], c0, c1, c2);
-    coeffs<T, CM, W>(g.ct, tc, 3, 1, c, g.C[CEYE], g.C[CEYHX], g.C[CEYHZ], c3, c4, c5);
-    coeffs<T, CM, W>(g.ct, tc, 3, 2, c, g.C[CEZE], g.C[CEZHY], g.C[CEZHX], c6, c7, c8);
+    coeffs<T, CM, W, NT>(g.ct, tc, 3, 0, c, g.C[CEXE], g.C[CEXHZ], g.C[CEXHY], c0, c1, c2);
+    coeffs<T, CM, W, NT>(g.ct, tc, 3, 1, c, g.C[CEYE], g.C[CEYHX], g.C[CEYHZ], c3, c4, c5);
+    coeffs<T, CM, W, NT>(g.ct, tc, 3, 2, c, g.C[CEZE], g.C[CEZHY], g.C[CEZHX], c6, c7, c8);
     // The six H differences of the curls, shared with the CPML terms (H does not change
     // during the E phase).
     V dy_ex, dz_ex, dz_ey, dx_ey, dx_ez, dy_ez;
@@ -668,9 +676,9 @@ __device__ __forceinline__ void update_e_cells(
         Ez[c + e] = ez[e];
       }
     } else {
-      if (i < nx) stv_stream<T, W>(Ex + c, ex);
-      stv_stream<T, W>(Ey + c, ey);
-      stv_stream<T, W>(Ez + c, ez);
+      if (i < nx) stv_stream<T, W, NT>(Ex + c, ex);
+      stv_stream<T, W, NT>(Ey + c, ey);
+      stv_stream<T, W, NT>(Ez + c, ez);
     }
 
     hz_m = hz_i;
@@ -693,7 +701,7 @@ __device__ __forceinline__ void update_e_cells(
 }
 
 // The stencil tile (col, ch) of the E bulk update (see update_h_body).
-template <typename T, int CM, int W, int FC>
+template <typename T, int CM, int W, int FC, bool NT = true>
 __device__ __forceinline__ void update_e_body(
     T *__restrict__ Ex, T *__restrict__ Ey, T *__restrict__ Ez,
     const T *__restrict__ Hx, const T *__restrict__ Hy, const T *__restrict__ Hz,
@@ -702,16 +710,16 @@ __device__ __forceinline__ void update_e_body(
   const int p = (col * 256 + (int)threadIdx.x) * W;
   const int j = p / g.SJ;
   if (j > g.ny) return;                                    // plane padding
-  update_e_cells<T, CM, W, FC>(Ex, Ey, Ez, Hx, Hy, Hz, g, tc, j, p - j * g.SJ, ch * m.chunk, m.chunk, ghost, fs);
+  update_e_cells<T, CM, W, FC, NT>(Ex, Ey, Ez, Hx, Hy, Hz, g, tc, j, p - j * g.SJ, ch * m.chunk, m.chunk, ghost, fs);
 }
 
-template <typename T, int CM, int W, int FC>
+template <typename T, int CM, int W, int FC, bool NT = true>
 __global__ void __launch_bounds__(256) k_update_e(Grid<T> g, MarchGeom m, int ghost, FaceSet<T> fs) {
   __shared__ TableCache<T, CM> tc;
   int col, ch;
   if (!march_tile(m, col, ch)) return;
   load_tables<T, CM>(g, 3, tc);
-  update_e_body<T, CM, W, FC>(g.Ex, g.Ey, g.Ez, g.Hx, g.Hy, g.Hz, g, m, tc, col, ch, ghost, fs);
+  update_e_body<T, CM, W, FC, NT>(g.Ex, g.Ey, g.Ez, g.Hx, g.Hy, g.Hz, g, m, tc, col, ch, ghost, fs);
 }
 
 // CPML_SPLIT_Z: one launch of the bulk update in which workgroups [0, z.ntiles) march
