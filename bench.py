@@ -336,7 +336,7 @@ def stencil_study(workload, precision, steps, warmup, timing_steps, workdir, ntf
     nh, ne = psi_counts(problem)
     cb = cell_phase_bytes(esz, coef_mode)
     dom = 0 if avg_ms[0] >= avg_ms[1] else 1
-    fused_axes = {0: "", 1: "z", 2: "xyz", 3: "z", 4: "xyz"}[tuning[3]]
+    fused_axes = {0: "", 1: "z", 2: "xyz", 3: "z", 4: "xyz", 5: "xy"}[tuning[3]]
     fh, fe = psi_counts(problem, fused_axes) if fused_axes else (0, 0)
     psi_fused = PSI_VALUES * esz * (fh if dom == 0 else fe)
     dom_bytes = cb * ncells + psi_fused
@@ -400,7 +400,8 @@ def stencil_study(workload, precision, steps, warmup, timing_steps, workdir, ntf
                    "coefficient_storage": ["indexed", "arrays"][tuning[2]],
                    "cpml": ["separate pass", "z slabs fused into bulk", "all faces fused into bulk",
                             "z slabs in their own workgroups of the bulk launch",
-                            "every slab cell in workgroups of its own in the bulk launch"][tuning[3]]},
+                            "every slab cell in workgroups of its own in the bulk launch",
+                            "x and y faces fused into bulk, z faces in the pass"][tuning[3]]},
     }
 
 

@@ -29,8 +29,12 @@ namespace fdtd {
 // SPLIT_ALL makes each phase one launch: the stencil tiles leave every slab cell alone,
 // and the CPML tiles of the separate pass's list, in workgroups of the same launch, give
 // each of their cells its bulk update and then its CPML terms (the E wall plane goes
-// with the cell of plane 0 at the same (j, k)).
-constexpr int CPML_SEPARATE = 0, CPML_FUSE_Z = 1, CPML_FUSE_ALL = 2, CPML_SPLIT_Z = 3, CPML_SPLIT_ALL = 4;
+// with the cell of plane 0 at the same (j, k)).  FUSE_XY applies the x- and y-face
+// terms inside the bulk kernels, right after each cell's bulk update (whole x planes
+// and whole rows share them, so a wave seldom diverges), and leaves the pass only the
+// z-face terms of the z-slab cells; z is the last face of every component's order.
+constexpr int CPML_SEPARATE = 0, CPML_FUSE_Z = 1, CPML_FUSE_ALL = 2, CPML_SPLIT_Z = 3, CPML_SPLIT_ALL = 4,
+              CPML_FUSE_XY = 5;
 
 // The tile list (build_cpml_tiles mode) the CPML pass uses under a placement.
 constexpr int cpml_tile_mode(int placement) {
@@ -98,7 +102,8 @@ inline std::vector<CpmlPiece> cpml_pieces(const CpmlLayout &L, int phase, int a)
 
 // The tiles of the cells the CPML pass owns for one phase and placement: every cell
 // of a face slab with CPML_SEPARATE; with CPML_FUSE_Z all but the z-slab cells of the
-// planes the bulk kernels cover (H: all; E: i < nx); none with CPML_FUSE_ALL.  Lanes
+// planes the bulk kernels cover (H: all; E: i < nx); none with CPML_FUSE_ALL; with
+// CPML_FUSE_XY the z-slab cells, with only their z-face bits in the mask.  Lanes
 // go along k (the box's k extent rounded up to a power of two, at most 64), then rows
 // along j (its j extent rounded up, up to what is left of 256), then planes along i.
 // Workgroups are dealt round-robin to the 8 XCDs, and tiles that read the same lines
@@ -129,9 +134,13 @@ inline void build_cpml_tiles(const CpmlLayout &L, int phase, int mode, std::vect
     for (const auto &py : P[1]) {
       std::vector<std::vector<std::vector<CpmlTile>>> chunks;   // per z piece, per (group, plane, k tile)
       for (const auto &pz : P[2]) {
-        const unsigned mask = px.mask | py.mask | pz.mask;
+        unsigned mask = px.mask | py.mask | pz.mask;
         if (!mask) continue;
         if (mode == CPML_FUSE_Z && pz.mask && px.hi <= i_end) continue;
+        if (mode == CPML_FUSE_XY) {
+          if (!pz.mask) continue;
+          mask = pz.mask;
+        }
         const int ks = log2_up(pz.hi - pz.lo, 6);
         const int js = log2_up(py.hi - py.lo, 8 - ks);
         const int kt = 1 << ks, jt = 1 << js, it = 256 >> (ks + js);

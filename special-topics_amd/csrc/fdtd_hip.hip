@@ -178,7 +178,7 @@ struct FdtdHipContext {
   CpmlTile *tile_buf = nullptr;
   size_t tile_cap = 0;
   int faces_version = 0, tiles_version = -1;
-  size_t tile_off[3][2] = {}, tile_n[3][2] = {};
+  size_t tile_off[CPML_FUSE_XY + 1][2] = {}, tile_n[CPML_FUSE_XY + 1][2] = {};
   const CpmlTile *cpml_tiles[2] = {};
   long ncpml[2] = {};
 
@@ -480,11 +480,11 @@ void launch_e_wn(FdtdHipContext *c, const Grid<T> &g, const MarchGeom &m, int gh
   else hipLaunchKernelGGL((k_update_e<T, CM, 1, FC, NT>), grid, block, 0, c->stream, g, m, ghost, fs);
 }
 
-// Plain (cached) accesses only with the separate CPML pass, the placement small grids
-// use; the fused placements always stream.
+// Plain (cached) accesses only with the separate CPML pass and with the x/y faces
+// fused, the placements small grids use; the others always stream.
 template <typename T, int CM, int FC>
 void launch_h_w(FdtdHipContext *c, const Grid<T> &g, const MarchGeom &m, const FaceSet<T> &fs) {
-  if constexpr (FC == CPML_SEPARATE) {
+  if constexpr (FC == CPML_SEPARATE || FC == CPML_FUSE_XY) {
     if (c->cache_resident) { launch_h_wn<T, CM, FC, false>(c, g, m, fs); return; }
   }
   launch_h_wn<T, CM, FC, true>(c, g, m, fs);
@@ -492,7 +492,7 @@ void launch_h_w(FdtdHipContext *c, const Grid<T> &g, const MarchGeom &m, const F
 
 template <typename T, int CM, int FC>
 void launch_e_w(FdtdHipContext *c, const Grid<T> &g, const MarchGeom &m, int ghost, const FaceSet<T> &fs) {
-  if constexpr (FC == CPML_SEPARATE) {
+  if constexpr (FC == CPML_SEPARATE || FC == CPML_FUSE_XY) {
     if (c->cache_resident) { launch_e_wn<T, CM, FC, false>(c, g, m, ghost, fs); return; }
   }
   launch_e_wn<T, CM, FC, true>(c, g, m, ghost, fs);
@@ -588,6 +588,7 @@ void launch_e_all(FdtdHipContext *c, const Grid<T> &g, const MarchGeom &m, int g
 template <typename T>
 void launch_h(FdtdHipContext *c, const Grid<T> &g, const MarchGeom &m, const FaceSet<T> &fs) {
   switch (fuse_cpml(c)) {
+    case CPML_FUSE_XY: launch_h_f<T, CPML_FUSE_XY>(c, g, m, fs); break;
     case CPML_SPLIT_ALL:
       if (c->coef_mode == COEF_U8) launch_h_all<T, COEF_U8>(c, g, m, fs);
       else if (c->coef_mode == COEF_U16) launch_h_all<T, COEF_U16>(c, g, m, fs);
@@ -607,6 +608,7 @@ void launch_h(FdtdHipContext *c, const Grid<T> &g, const MarchGeom &m, const Fac
 template <typename T>
 void launch_e(FdtdHipContext *c, const Grid<T> &g, const MarchGeom &m, int ghost, const FaceSet<T> &fs) {
   switch (fuse_cpml(c)) {
+    case CPML_FUSE_XY: launch_e_f<T, CPML_FUSE_XY>(c, g, m, ghost, fs); break;
     case CPML_SPLIT_ALL:
       if (c->coef_mode == COEF_U8) launch_e_all<T, COEF_U8>(c, g, m, ghost, fs);
       else if (c->coef_mode == COEF_U16) launch_e_all<T, COEF_U16>(c, g, m, ghost, fs);
@@ -631,7 +633,7 @@ int set_cpml_tiles(FdtdHipContext *c) {
       return fail("CPML pass: grid axes of more than %d cells are not supported", CPML_TILE_MAX_AXIS);
     const CpmlLayout L = cpml_layout(c);
     std::vector<CpmlTile> all;
-    for (int mode = 0; mode < 3; ++mode)
+    for (int mode : {CPML_SEPARATE, CPML_FUSE_Z, CPML_FUSE_ALL, CPML_FUSE_XY})
       for (int ph = 0; ph < 2; ++ph) {
         c->tile_off[mode][ph] = all.size();
         build_cpml_tiles(L, ph, mode, all);
@@ -1468,8 +1470,8 @@ static int check_tuning(const FdtdHipContext *c, const FdtdHipTuning *t, const c
     return fail("%s: %d cells per thread unsupported (1, 2%s)", who, t->cells_per_thread, maxw == 4 ? " or 4" : "");
   if (t->coefficient_storage < 0 || t->coefficient_storage > 1)
     return fail("%s: coefficient storage %d unknown", who, t->coefficient_storage);
-  if (t->fuse_cpml < CPML_SEPARATE || t->fuse_cpml > CPML_SPLIT_ALL)
-    return fail("%s: fuse_cpml %d unknown (0 to 4)", who, t->fuse_cpml);
+  if (t->fuse_cpml < CPML_SEPARATE || t->fuse_cpml > CPML_FUSE_XY)
+    return fail("%s: fuse_cpml %d unknown (0 to 5)", who, t->fuse_cpml);
   return FDTD_HIP_OK;
 }
 
@@ -1530,7 +1532,7 @@ int fdtd_hip_autotune(FdtdHipContext *c, int steps_per_candidate, FdtdHipTuning 
     for (int w : ws) {
       if (w == 4 && !c->fp32()) continue;
       for (int pl : planes)
-        for (int fc = CPML_SEPARATE; fc <= CPML_SPLIT_ALL; ++fc) cands.push_back(FdtdHipTuning{pl, w, cs, fc});
+        for (int fc = CPML_SEPARATE; fc <= CPML_FUSE_XY; ++fc) cands.push_back(FdtdHipTuning{pl, w, cs, fc});
     }
   }
   double best = 1e300;
@@ -1857,6 +1859,7 @@ void multi_launch_hf(FdtdHipMulti *m, bool e_phase) {
 template <typename T>
 void multi_launch_phase(FdtdHipMulti *m, bool e_phase) {
   switch (m->tune.fuse_cpml) {
+    case CPML_FUSE_XY: multi_launch_hf<T, CPML_FUSE_XY>(m, e_phase); break;
     case CPML_FUSE_ALL: multi_launch_hf<T, CPML_FUSE_ALL>(m, e_phase); break;
     case CPML_FUSE_Z: multi_launch_hf<T, CPML_FUSE_Z>(m, e_phase); break;
     default: multi_launch_hf<T, CPML_SEPARATE>(m, e_phase); break;
@@ -2085,7 +2088,8 @@ int fdtd_hip_multi_autotune(FdtdHipMulti *m, int steps_per_candidate, FdtdHipTun
     for (int w : {1, 2, 4}) {
       if (w == 4 && !m->fp32) continue;
       for (int pl : {1, 2, 4, 8})
-        for (int fc = CPML_SEPARATE; fc <= CPML_FUSE_ALL; ++fc) cands.push_back(FdtdHipTuning{pl, w, cs, fc});
+        for (int fc : {CPML_SEPARATE, CPML_FUSE_Z, CPML_FUSE_ALL, CPML_FUSE_XY})
+          cands.push_back(FdtdHipTuning{pl, w, cs, fc});
     }
   hipEvent_t e0 = nullptr, e1 = nullptr;
   HIP_TRY(hipSetDevice(m->device));

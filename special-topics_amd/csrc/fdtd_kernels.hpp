@@ -218,11 +218,14 @@ __device__ __forceinline__ T cpml_step(const FaceSet<T> &fs, int s, int ci, T va
 
 // One component's CPML terms on the value v: faces of axis A1 (n then p) then of axis
 // A2, each with its curl difference computed on demand.
-template <typename T, int A1, int S1, int A2, int S2, bool MAG, typename D1, typename D2>
+// NOZ leaves out the z faces (CPML_FUSE_XY: the separate pass applies them after).
+template <typename T, int A1, int S1, int A2, int S2, bool MAG, bool NOZ = false, typename D1, typename D2>
 __device__ __forceinline__ T cpml_terms(const FaceSet<T> &fs, T v, int i, int j, int k, int d1, int d2,
                                         D1 diff1, D2 diff2) {
+  constexpr bool SKIP2 = NOZ && A2 == 2;
   const int a0 = cpml_slot<T, A1, MAG>(fs, 0, i, j, k), a1 = cpml_slot<T, A1, MAG>(fs, 1, i, j, k);
-  const int b0 = cpml_slot<T, A2, MAG>(fs, 0, i, j, k), b1 = cpml_slot<T, A2, MAG>(fs, 1, i, j, k);
+  const int b0 = SKIP2 ? -1 : cpml_slot<T, A2, MAG>(fs, 0, i, j, k);
+  const int b1 = SKIP2 ? -1 : cpml_slot<T, A2, MAG>(fs, 1, i, j, k);
   if (a0 >= 0 || a1 >= 0) {
     const T d = diff1();
     if (a0 >= 0) v = cpml_step<T, A1, S1, MAG>(fs, 0, a0, v, d, i, j, k, d1, d2);
@@ -249,6 +252,7 @@ __device__ __forceinline__ bool fused_cpml_cell(const FaceSet<T> &fs, int i, int
   if (FC == CPML_FUSE_Z) return in_slab<T, 2, MAG>(fs, k);
   if (FC == CPML_FUSE_ALL)
     return in_slab<T, 0, MAG>(fs, i) || in_slab<T, 1, MAG>(fs, j) || in_slab<T, 2, MAG>(fs, k);
+  if (FC == CPML_FUSE_XY) return in_slab<T, 0, MAG>(fs, i) || in_slab<T, 1, MAG>(fs, j);
   return false;
 }
 
@@ -468,21 +472,23 @@ __device__ __forceinline__ void update_h_cells(
         hz[e] = h;
       }
     }
-    if (FC == CPML_FUSE_Z || FC == CPML_FUSE_ALL) {
-      // Cells the bulk kernel owns get all their CPML terms here, right after their
-      // bulk update and in the reference's face order; k_cpml skips them.
+    if (FC == CPML_FUSE_Z || FC == CPML_FUSE_ALL || FC == CPML_FUSE_XY) {
+      // Cells the bulk kernel owns get all their CPML terms here (CPML_FUSE_XY: those of
+      // the x and y faces), right after their bulk update and in the reference's face
+      // order; k_cpml skips them (CPML_FUSE_XY: applies the z-face terms after).
+      constexpr bool NOZ = FC == CPML_FUSE_XY;
 #pragma unroll
       for (int e = 0; e < W; ++e) {
         const int k = k0 + e;
         if (k > nz || !fused_cpml_cell<T, FC, true>(fs, i, j, k)) continue;
         if (j < ny && k < nz)                              // Hx: y then z faces
-          hx[e] = cpml_terms<T, 1, 0, 2, 0, true>(fs, hx[e], i, j, k, ny, nz,
+          hx[e] = cpml_terms<T, 1, 0, 2, 0, true, NOZ>(fs, hx[e], i, j, k, ny, nz,
               [&] { return dy_hx[e]; }, [&] { return dz_hx[e]; });
         if (inner && k < nz)                               // Hy: x then z faces
-          hy[e] = cpml_terms<T, 0, 0, 2, 1, true>(fs, hy[e], i, j, k, ny + 1, nz,
+          hy[e] = cpml_terms<T, 0, 0, 2, 1, true, NOZ>(fs, hy[e], i, j, k, ny + 1, nz,
               [&] { return dx_hy[e]; }, [&] { return dz_hy[e]; });
         if (inner && j < ny)                               // Hz: x then y faces
-          hz[e] = cpml_terms<T, 0, 1, 1, 1, true>(fs, hz[e], i, j, k, ny, nz + 1,
+          hz[e] = cpml_terms<T, 0, 1, 1, 1, true, NOZ>(fs, hz[e], i, j, k, ny, nz + 1,
               [&] { return dx_hz[e]; }, [&] { return dy_hz[e]; });
       }
     }
@@ -552,20 +558,20 @@ __device__ __forceinline__ void e_wall_bulk(const Grid<T> &g, T *__restrict__ Ey
 }
 
 // CPML terms of the i = nx wall's Ey/Ez at (j, k), after its bulk update (or none,
-// under the matlab boundary), in the reference's face order.
-template <typename T>
+// under the matlab boundary), in the reference's face order (NOZ: without the z faces).
+template <typename T, bool NOZ = false>
 __device__ __forceinline__ void e_wall_terms(const Grid<T> &g, const FaceSet<T> &fs, T *__restrict__ Ey,
                                              T *__restrict__ Ez, const T *__restrict__ Hx, const T *__restrict__ Hy,
                                              const T *__restrict__ Hz, int j, int k) {
 #pragma clang fp contract(off)
   const int nx = g.nx, ny = g.ny, nz = g.nz, SJ = g.SJ, SI = g.SI;
   const int ce = nx * SI + j * SJ + k;
-  if (k > nz || !fused_cpml_cell<T, CPML_FUSE_ALL, false>(fs, nx, j, k)) return;
+  if (k > nz || !fused_cpml_cell<T, NOZ ? CPML_FUSE_XY : CPML_FUSE_ALL, false>(fs, nx, j, k)) return;
   if (j < ny)                                            // Ey(nx): x then z faces
-    Ey[ce] = cpml_terms<T, 0, 2, 2, 3, false>(fs, Ey[ce], nx, j, k, ny, nz + 1,
+    Ey[ce] = cpml_terms<T, 0, 2, 2, 3, false, NOZ>(fs, Ey[ce], nx, j, k, ny, nz + 1,
         [&] { return Hz[ce] - Hz[ce - SI]; }, [&] { return Hx[ce] - Hx[ce - 1]; });
   if (k < nz)                                            // Ez(nx): x then y faces
-    Ez[ce] = cpml_terms<T, 0, 3, 1, 2, false>(fs, Ez[ce], nx, j, k, ny + 1, nz,
+    Ez[ce] = cpml_terms<T, 0, 3, 1, 2, false, NOZ>(fs, Ez[ce], nx, j, k, ny + 1, nz,
         [&] { return Hy[ce] - Hy[ce - SI]; }, [&] { return Hx[ce] - Hx[ce - SJ]; });
 }
 
@@ -650,20 +656,21 @@ __device__ __forceinline__ void update_e_cells(
       }
     }
     if (i == 0) ex0 = ex;
-    if (FC == CPML_FUSE_Z || FC == CPML_FUSE_ALL) {
-      // As in update_h_body; ex0 keeps the pre-CPML Ex that the ghost plane reads.
+    if (FC == CPML_FUSE_Z || FC == CPML_FUSE_ALL || FC == CPML_FUSE_XY) {
+      // As in update_h_cells; ex0 keeps the pre-CPML Ex that the ghost plane reads.
+      constexpr bool NOZ = FC == CPML_FUSE_XY;
 #pragma unroll
       for (int e = 0; e < W; ++e) {
         const int k = k0 + e;
         if (k > nz || !fused_cpml_cell<T, FC, false>(fs, i, j, k)) continue;
         if (i < nx)                                        // Ex: y then z faces
-          ex[e] = cpml_terms<T, 1, 3, 2, 2, false>(fs, ex[e], i, j, k, ny + 1, nz + 1,
+          ex[e] = cpml_terms<T, 1, 3, 2, 2, false, NOZ>(fs, ex[e], i, j, k, ny + 1, nz + 1,
               [&] { return dy_ex[e]; }, [&] { return dz_ex[e]; });
         if (j < ny)                                        // Ey: x then z faces
-          ey[e] = cpml_terms<T, 0, 2, 2, 3, false>(fs, ey[e], i, j, k, ny, nz + 1,
+          ey[e] = cpml_terms<T, 0, 2, 2, 3, false, NOZ>(fs, ey[e], i, j, k, ny, nz + 1,
               [&] { return dx_ey[e]; }, [&] { return dz_ey[e]; });
         if (k < nz)                                        // Ez: x then y faces
-          ez[e] = cpml_terms<T, 0, 3, 1, 2, false>(fs, ez[e], i, j, k, ny + 1, nz,
+          ez[e] = cpml_terms<T, 0, 3, 1, 2, false, NOZ>(fs, ez[e], i, j, k, ny + 1, nz,
               [&] { return dx_ez[e]; }, [&] { return dy_ez[e]; });
       }
     }
@@ -689,13 +696,13 @@ __device__ __forceinline__ void update_e_cells(
   // the march; ex0 is Ex(0) as the bulk left it.  Kept out of the march loop, its
   // registers do not add to the loop's.  With CPML_FUSE_ALL / CPML_SPLIT_ALL the same
   // thread then applies the wall's CPML terms.
-  constexpr bool WALL_TERMS = FC == CPML_FUSE_ALL || FC == CPML_SPLIT_ALL;
+  constexpr bool WALL_TERMS = FC == CPML_FUSE_ALL || FC == CPML_SPLIT_ALL || FC == CPML_FUSE_XY;
   if ((ghost || WALL_TERMS) && i0 == 0) {
 #pragma unroll
     for (int e = 0; e < W; ++e) {
       if (skip0 >> e & 1u) continue;
       if (ghost) e_wall_bulk<T>(g, Ey, Ez, Hx, Hy, Hz, j, k0 + e, ex0[e]);
-      if (WALL_TERMS) e_wall_terms<T>(g, fs, Ey, Ez, Hx, Hy, Hz, j, k0 + e);
+      if (WALL_TERMS) e_wall_terms<T, FC == CPML_FUSE_XY>(g, fs, Ey, Ez, Hx, Hy, Hz, j, k0 + e);
     }
   }
 }

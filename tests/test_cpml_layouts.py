@@ -48,7 +48,7 @@ def _present(problem):
 
 
 @pytest.mark.parametrize("boundary", ["reference", "matlab"])
-@pytest.mark.parametrize("fuse", [0, 1, 2, 3, 4])
+@pytest.mark.parametrize("fuse", [0, 1, 2, 3, 4, 5])
 @pytest.mark.parametrize("layout", sorted(LAYOUTS))
 def test_uneven_layout_fp64_matches_oracle(get_problem, layouts, layout, fuse, boundary):
     from fdtdhip.sim import HipSim

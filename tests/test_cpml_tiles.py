@@ -5,7 +5,7 @@ updateMagneticCPML / updateElectricCPML loops visit (js_fdtd/src/cpml.js:456-100
 minus those a fused placement leaves to the bulk kernels.  cpml_tiles_check.cpp
 decodes every lane of every tile the way the kernels do and compares with a brute-force
 enumeration of the grid, for standard and seeded random slab layouts, both phases and
-all three placements.  It also checks the pass's tile order at 256^3: each plane's
+every tile list (separate pass; z slabs, all faces or x/y faces fused).  It also checks the pass's tile order at 256^3: each plane's
 low-z tiles are followed, one XCD round later, by the high-z tiles of the same rows.
 Host-only: plain g++ on the header.
 """

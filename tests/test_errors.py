@@ -88,7 +88,7 @@ def test_tuning_and_field_reads_are_checked(get_problem):
     sim = _sim(prob)
     _raises("cells per thread", sim.set_tuning, 0, 4, 0, 0)       # 4 cells is FP32 only
     _raises("cells per thread", sim.set_tuning, 0, 3, 0, 0)
-    _raises("fuse_cpml", sim.set_tuning, 0, 1, 0, 5)
+    _raises("fuse_cpml", sim.set_tuning, 0, 1, 0, 6)
     _raises("coefficient storage", sim.set_tuning, 0, 1, 7, 0)
     out = np.zeros(3)
     _raises("elements", lambda: native.check(sim.L.fdtd_hip_read_field(

@@ -41,14 +41,14 @@ def _check_run_golden(sim, fx, fields=True):
     return v, i
 
 
-@pytest.mark.parametrize("fuse", [0, 1, 2, 3, 4])
+@pytest.mark.parametrize("fuse", [0, 1, 2, 3, 4, 5])
 @pytest.mark.parametrize("storage,expect", [(0, 1), (1, 0)])
 @pytest.mark.parametrize("cells", [1, 2])
 def test_golden_small_fp64_matches_reference_run(get_problem, storage, expect, cells, fuse):
     """Every coefficient storage (deduplicated indices, per-cell arrays), launch shape
     and CPML placement (separate pass, z slabs fused into the bulk kernels, every face
     fused, z slabs in workgroups of their own, every slab cell in workgroups of their
-    own) reproduces the reference run bit for bit."""
+    own, x and y faces fused) reproduces the reference run bit for bit."""
     prob = get_problem("golden-small")
     fx = G.load("run-golden-small.json")
     sim = _sim(prob)
@@ -69,7 +69,7 @@ def test_march_depth_does_not_change_results(get_problem, planes, cells):
     sim.close()
 
 
-@pytest.mark.parametrize("fuse", [0, 1, 2, 3, 4])
+@pytest.mark.parametrize("fuse", [0, 1, 2, 3, 4, 5])
 @pytest.mark.parametrize("cells", [1, 2, 4])
 def test_fp32_launch_shapes_agree(get_problem, cells, fuse):
     """FP32 results do not depend on the launch shape or on where the z-slab CPML
@@ -92,7 +92,7 @@ def test_autotune_then_run_matches_reference(get_problem):
     fx = G.load("run-golden-small.json")
     sim = _sim(prob)
     planes, cells, storage, fuse = sim.autotune(2)
-    assert planes >= 1 and cells in (1, 2) and storage in (0, 1) and fuse in (0, 1, 2, 3, 4)
+    assert planes >= 1 and cells in (1, 2) and storage in (0, 1) and fuse in (0, 1, 2, 3, 4, 5)
     _check_run_golden(sim, fx)
     sim.close()
 
@@ -120,7 +120,7 @@ def test_random_state_one_step_fp64_matches_oracle(get_problem):
     sim.close()
 
 
-@pytest.mark.parametrize("fuse", [0, 1, 2, 3, 4])
+@pytest.mark.parametrize("fuse", [0, 1, 2, 3, 4, 5])
 def test_matlab_boundary_fp64_matches_oracle(get_problem, fuse):
     prob = get_problem("golden-small")
     sim = _sim(prob, boundary="matlab")
@@ -151,7 +151,8 @@ def test_batches_compose(get_problem):
                                            ("ifa-matlab-7000", 3), ("ifa-config1-7000", 0),
                                            ("ifa-config1-7000", 1), ("ifa-config1-7000", 2),
                                            ("ifa-config1-7000", 3), ("ifa-config1-7000", 4),
-                                           ("ifa-matlab-7000", 4), ("ifa-fr4-substrate", 0),
+                                           ("ifa-matlab-7000", 4), ("ifa-config1-7000", 5),
+                                           ("ifa-matlab-7000", 5), ("ifa-fr4-substrate", 0),
                                            ("ifa-finefreq-sweep", 0)])
 def test_ifa_fp64_full_run_matches_reference(get_problem, scenario, fuse):
     """The 39x136x136 IFA scenarios (7000 FP64 steps; 300 for the FR4-substrate and

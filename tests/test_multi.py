@@ -67,7 +67,8 @@ def _assert_same(a, b, label):
 
 
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
-@pytest.mark.parametrize("cells,graphs,fuse", [(1, True, 2), (2, False, 2), (1, True, 0), (2, True, 1)])
+@pytest.mark.parametrize("cells,graphs,fuse", [(1, True, 2), (2, False, 2), (1, True, 0), (2, True, 1),
+                                              (2, True, 5)])
 def test_batched_run_matches_each_single_run(variants, precision, cells, graphs, fuse):
     from fdtdhip.sim import HipMulti
     probs = variants[precision == "fp32"]
