@@ -259,7 +259,9 @@ typedef struct {
                                   workgroups of their own in the bulk kernels' launch;
                                   4: every CPML cell is, so each phase is one launch
                                   and there is no separate pass (3 and 4: single
-                                  problems only, not fdtd_hip_multi_*) */
+                                  problems only, not fdtd_hip_multi_*); 5: the x- and
+                                  y-face terms inside the bulk kernels, the z-face
+                                  terms in the separate pass */
 } FdtdHipTuning;
 
 int fdtd_hip_set_tuning(FdtdHipContext *ctx, const FdtdHipTuning *tuning);
