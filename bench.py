@@ -345,13 +345,20 @@ def stencil_study(workload, precision, steps, warmup, timing_steps, workdir, ntf
     achieved = dom_bytes / dom_s / 1e9
 
     ctype = "float" if fp32 else "double"
+    # Six field arrays of at most 64 MiB: the library's stencils then use plain (cached)
+    # loads and stores under placements 0 and 5 (the kernels' last template argument).
+    sj = (nz + 1 + 3) // 4 * 4
+    si = ((ny + 1) * sj + 1023) // 1024 * 1024
+    stride = ((nx + 3) * si * esz + 255) // 256 * 256
+    nt = "false" if 6 * stride <= 64 << 20 and tuning[3] in (0, 5) else "true"
+
     def stencil_name(phase):
         w = max(tuning[1], 1)
         if tuning[3] == 4:          # every slab cell in workgroups of their own
             return f"fdtd::k_update_{phase}_all<{ctype}, {coef_mode}, {w}>"
         if tuning[3] == 3:          # z slabs in workgroups of their own: the split kernels
             return f"fdtd::k_update_{phase}_split<{ctype}, {coef_mode}, {w}>"
-        return f"fdtd::k_update_{phase}<{ctype}, {coef_mode}, {w}, {tuning[3]}>"
+        return f"fdtd::k_update_{phase}<{ctype}, {coef_mode}, {w}, {tuning[3]}, {nt}>"
     kname = stencil_name("h" if dom == 0 else "e")
     table = traffic_table(workload, precision)
     traffic = table.get(kname)
