@@ -102,7 +102,9 @@ def test_batched_autotune_then_run_matches_single(variants):
     sims = _sims(probs, "fp32")
     multi = HipMulti(sims)
     planes, cells, storage, fuse = multi.autotune(2)
-    assert planes >= 1 and cells in (1, 2, 4) and storage in (0, 1) and fuse in (0, 1, 2)
+    # The batch autotuner times placements 0, 1, 2 and 5 (placements 3 and 4 are
+    # single-problem launches).
+    assert planes >= 1 and cells in (1, 2, 4) and storage in (0, 1) and fuse in (0, 1, 2, 5)
     got = multi.run(0, STEPS)
     states = [_state(s, [got[n]]) for n, s in enumerate(sims)]
     multi.close()
