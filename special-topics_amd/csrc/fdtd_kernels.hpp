@@ -239,6 +239,82 @@ __device__ __forceinline__ T cpml_terms(const FaceSet<T> &fs, T v, int i, int j,
   return v;
 }
 
+// The terms of the faces along axis A for the W cells of a stencil vector, for the two
+// components P and Q of the phase whose terms include that axis (slots SP and SQ of the
+// face; own extents p1 x p2 and q1 x q2).  Bit e of okp / okq says that cell e of P / Q
+// takes its terms here.  Staged: every Psi, CPsi and profile load of the W cells and
+// both components is issued first, then the recurrences, corrections and Psi stores
+// follow.  (Per term, load-store-load would cost two dependent round trips, chained
+// across cells: the compiler may not move a CPsi load above a Psi store.)
+// A lane stages one side of the axis: the n side when one of its cells lies in the n
+// slab, else the p side.  Its cells' Psi elements and (along z) profile entries are then
+// consecutive, base + e.  p-side terms of a lane that staged the n side (a vector that
+// reaches both slabs, or a cell in both: an axis shorter than its two slabs) follow,
+// unstaged, after the n side's.  The components are independent, and each still takes
+// its faces in the reference's order (cpml.js:456-1003) when the axes are applied x, then
+// y, then z.
+template <typename T, int W, int A, bool MAG, int SP, int SQ>
+__device__ __forceinline__ void fused_axis_terms(const FaceSet<T> &fs, int i, int j, int k0, Vec<T, W> &vp,
+                                                 Vec<T, W> &vq, const Vec<T, W> &dp, const Vec<T, W> &dq,
+                                                 unsigned okp, unsigned okq, int p1, int p2, int q1, int q2) {
+#pragma clang fp contract(off)
+  unsigned inn = 0, inp = 0;
+#pragma unroll
+  for (int e = 0; e < W; ++e) {
+    if (cpml_slot<T, A, MAG>(fs, 0, i, j, k0 + e) >= 0) inn |= 1u << e;
+    if (cpml_slot<T, A, MAG>(fs, 1, i, j, k0 + e) >= 0) inp |= 1u << e;
+  }
+  const unsigned ok = okp | okq;
+  if (!((inn | inp) & ok)) return;
+  const bool s = !(inn & ok);                              // the side this lane stages
+  const unsigned fast = (s ? inp : inn) & ok, slow = s ? 0u : inp & ok;
+  const Face<T> &F = fs.f[2 * A + (s ? 1 : 0)];
+  const int ci0 = (A == 0 ? i : (A == 1 ? j : k0)) - (MAG ? F.m_start : F.e_start);
+  const int qp0 = psi_offset<A>(i, j, k0, ci0, F.nc, p1, p2), qq0 = psi_offset<A>(i, j, k0, ci0, F.nc, q1, q2);
+  T *__restrict__ psP = F.psi[SP] + qp0;
+  T *__restrict__ psQ = F.psi[SQ] + qq0;
+  const T *__restrict__ cpP = F.cpsi[SP] + qp0;
+  const T *__restrict__ cpQ = F.cpsi[SQ] + qq0;
+  const T *__restrict__ bb = (MAG ? F.b_m : F.b_e) + ci0;
+  const T *__restrict__ aa = (MAG ? F.a_m : F.a_e) + ci0;
+  constexpr int NB = A == 2 ? W : 1;                       // profile entries per lane
+  T psp[W], cpp[W], psq[W], cpq[W], b[NB], a[NB];
+#pragma unroll
+  for (int e = 0; e < W; ++e) {
+    if (!(fast >> e & 1u)) continue;
+    if (A == 2 || !(fast & ((1u << e) - 1u))) {            // along z every cell, else the first
+      b[A == 2 ? e : 0] = bb[A == 2 ? e : 0];
+      a[A == 2 ? e : 0] = aa[A == 2 ? e : 0];
+    }
+    if (okp >> e & 1u) { psp[e] = psP[e]; cpp[e] = cpP[e]; }
+    if (okq >> e & 1u) { psq[e] = psQ[e]; cpq[e] = cpQ[e]; }
+  }
+#pragma unroll
+  for (int e = 0; e < W; ++e) {
+    if (!(fast >> e & 1u)) continue;
+    const T be = b[A == 2 ? e : 0], ae = a[A == 2 ? e : 0];
+    if (okp >> e & 1u) {
+      const T ps = be * psp[e] + ae * dp[e];
+      psP[e] = ps;
+      vp[e] = vp[e] + cpp[e] * ps;
+    }
+    if (okq >> e & 1u) {
+      const T ps = be * psq[e] + ae * dq[e];
+      psQ[e] = ps;
+      vq[e] = vq[e] + cpq[e] * ps;
+    }
+  }
+  if (slow) {
+#pragma unroll
+    for (int e = 0; e < W; ++e) {
+      if (!(slow >> e & 1u)) continue;
+      const int cq = cpml_slot<T, A, MAG>(fs, 1, i, j, k0 + e);
+      if (okp >> e & 1u) vp[e] = cpml_step<T, A, SP, MAG>(fs, 1, cq, vp[e], dp[e], i, j, k0 + e, p1, p2);
+      if (okq >> e & 1u) vq[e] = cpml_step<T, A, SQ, MAG>(fs, 1, cq, vq[e], dq[e], i, j, k0 + e, q1, q2);
+    }
+  }
+}
+
 // Whether coordinate v along AXIS lies in a slab of that axis's faces (MAG: H planes,
 // else E planes).
 template <typename T, int AXIS, bool MAG>
@@ -422,6 +498,8 @@ __device__ __forceinline__ void update_h_cells(
   const int p = j * SJ + k0;
   const int i1 = min(i0 + chunk, nx + 1);
   constexpr unsigned ALL = (1u << W) - 1;
+  constexpr bool FUSED = FC == CPML_FUSE_Z || FC == CPML_FUSE_ALL || FC == CPML_FUSE_XY;
+  constexpr bool NOZ = FC == CPML_FUSE_XY;
   // CPML_SPLIT_Z / CPML_SPLIT_ALL: cells of the slabs these placements hand to other
   // workgroups of the launch are neither updated nor stored here.
   const unsigned row_skip = split_row_bits<T, W, FC, true>(fs, j, k0);
@@ -472,24 +550,25 @@ __device__ __forceinline__ void update_h_cells(
         hz[e] = h;
       }
     }
-    if (FC == CPML_FUSE_Z || FC == CPML_FUSE_ALL || FC == CPML_FUSE_XY) {
-      // Cells the bulk kernel owns get all their CPML terms here (CPML_FUSE_XY: those of
-      // the x and y faces), right after their bulk update and in the reference's face
-      // order; k_cpml skips them (CPML_FUSE_XY: applies the z-face terms after).
-      constexpr bool NOZ = FC == CPML_FUSE_XY;
+    if (FUSED) {
+      // Cells the bulk kernel owns get their CPML terms here (CPML_FUSE_XY: those of the
+      // x and y faces), right after their bulk update and in the reference's face order;
+      // k_cpml skips them (CPML_FUSE_XY: applies the z-face terms after).  Hx takes its y
+      // then z faces, Hy x then z, Hz x then y: one staged round per axis.
+      unsigned okx = 0, oky = 0, okz = 0;
 #pragma unroll
       for (int e = 0; e < W; ++e) {
         const int k = k0 + e;
         if (k > nz || !fused_cpml_cell<T, FC, true>(fs, i, j, k)) continue;
-        if (j < ny && k < nz)                              // Hx: y then z faces
-          hx[e] = cpml_terms<T, 1, 0, 2, 0, true, NOZ>(fs, hx[e], i, j, k, ny, nz,
-              [&] { return dy_hx[e]; }, [&] { return dz_hx[e]; });
-        if (inner && k < nz)                               // Hy: x then z faces
-          hy[e] = cpml_terms<T, 0, 0, 2, 1, true, NOZ>(fs, hy[e], i, j, k, ny + 1, nz,
-              [&] { return dx_hy[e]; }, [&] { return dz_hy[e]; });
-        if (inner && j < ny)                               // Hz: x then y faces
-          hz[e] = cpml_terms<T, 0, 1, 1, 1, true, NOZ>(fs, hz[e], i, j, k, ny, nz + 1,
-              [&] { return dx_hz[e]; }, [&] { return dy_hz[e]; });
+        if (j < ny && k < nz) okx |= 1u << e;
+        if (inner && k < nz) oky |= 1u << e;
+        if (inner && j < ny) okz |= 1u << e;
+      }
+      if (okx | oky | okz) {
+        fused_axis_terms<T, W, 0, true, 0, 1>(fs, i, j, k0, hy, hz, dx_hy, dx_hz, oky, okz, ny + 1, nz, ny, nz + 1);
+        fused_axis_terms<T, W, 1, true, 0, 1>(fs, i, j, k0, hx, hz, dy_hx, dy_hz, okx, okz, ny, nz, ny, nz + 1);
+        if (!NOZ)
+          fused_axis_terms<T, W, 2, true, 0, 1>(fs, i, j, k0, hx, hy, dz_hx, dz_hy, okx, oky, ny, nz, ny + 1, nz);
       }
     }
     if (skip) {
@@ -595,6 +674,8 @@ __device__ __forceinline__ void update_e_cells(
   const int p = j * SJ + k0;
   const int i1 = min(i0 + chunk, nx);
   constexpr unsigned ALL = (1u << W) - 1;
+  constexpr bool FUSED = FC == CPML_FUSE_Z || FC == CPML_FUSE_ALL || FC == CPML_FUSE_XY;
+  constexpr bool NOZ = FC == CPML_FUSE_XY;
   // CPML_SPLIT_Z / CPML_SPLIT_ALL: cells (wall cells included) of the slabs these
   // placements hand to other workgroups of the launch are left alone here.
   const unsigned row_skip = split_row_bits<T, W, FC, false>(fs, j, k0);
@@ -656,22 +737,25 @@ __device__ __forceinline__ void update_e_cells(
       }
     }
     if (i == 0) ex0 = ex;
-    if (FC == CPML_FUSE_Z || FC == CPML_FUSE_ALL || FC == CPML_FUSE_XY) {
-      // As in update_h_cells; ex0 keeps the pre-CPML Ex that the ghost plane reads.
-      constexpr bool NOZ = FC == CPML_FUSE_XY;
+    if (FUSED) {
+      // As in update_h_cells; ex0 keeps the pre-CPML Ex that the ghost plane reads.  Ex
+      // takes its y then z faces, Ey x then z, Ez x then y.
+      unsigned okx = 0, oky = 0, okz = 0;
 #pragma unroll
       for (int e = 0; e < W; ++e) {
         const int k = k0 + e;
         if (k > nz || !fused_cpml_cell<T, FC, false>(fs, i, j, k)) continue;
-        if (i < nx)                                        // Ex: y then z faces
-          ex[e] = cpml_terms<T, 1, 3, 2, 2, false, NOZ>(fs, ex[e], i, j, k, ny + 1, nz + 1,
-              [&] { return dy_ex[e]; }, [&] { return dz_ex[e]; });
-        if (j < ny)                                        // Ey: x then z faces
-          ey[e] = cpml_terms<T, 0, 2, 2, 3, false, NOZ>(fs, ey[e], i, j, k, ny, nz + 1,
-              [&] { return dx_ey[e]; }, [&] { return dz_ey[e]; });
-        if (k < nz)                                        // Ez: x then y faces
-          ez[e] = cpml_terms<T, 0, 3, 1, 2, false, NOZ>(fs, ez[e], i, j, k, ny + 1, nz,
-              [&] { return dx_ez[e]; }, [&] { return dy_ez[e]; });
+        if (i < nx) okx |= 1u << e;
+        if (j < ny) oky |= 1u << e;
+        if (k < nz) okz |= 1u << e;
+      }
+      if (okx | oky | okz) {
+        fused_axis_terms<T, W, 0, false, 2, 3>(fs, i, j, k0, ey, ez, dx_ey, dx_ez, oky, okz, ny, nz + 1, ny + 1, nz);
+        fused_axis_terms<T, W, 1, false, 2, 3>(fs, i, j, k0, ez, ex, dy_ez, dy_ex, okz, okx, ny + 1, nz, ny + 1,
+                                               nz + 1);
+        if (!NOZ)
+          fused_axis_terms<T, W, 2, false, 2, 3>(fs, i, j, k0, ex, ey, dz_ex, dz_ey, okx, oky, ny + 1, nz + 1, ny,
+                                                 nz + 1);
       }
     }
     if (skip) {
