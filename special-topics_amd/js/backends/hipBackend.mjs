@@ -67,6 +67,15 @@ function asFloat64(a) {
   return a instanceof Float64Array ? a : Float64Array.from(a);
 }
 
+// FP32 under the reference boundary: the i = nx ghost-plane update makes long IFA runs
+// grow without bound after about step 5000 (js_fdtd/scenarios/ifa-matlab-7000.json:3),
+// and the growth amplifies FP32 rounding along with everything else, so such a run's S11
+// can differ from the FP64 run by tens of dB.  Said once per process.
+const FP32_REFERENCE_WARNING = "hip backend: precision 'fp32' with boundary 'reference': the reference's " +
+  "i = nx ghost-plane update makes long runs diverge and amplifies FP32 rounding; use boundary 'matlab' " +
+  "for physical FP32 results, or precision 'fp64' for the reference's own numbers";
+let warnedFp32Reference = false;
+
 /** Backend options: problem.options first, then FDTD_HIP_* environment variables. */
 export function resolveOptions(options) {
   const o = options || {};
@@ -75,6 +84,10 @@ export function resolveOptions(options) {
   const boundary = String(pick(o.boundary, env.FDTD_HIP_BOUNDARY || 'reference')).toLowerCase();
   if (!(precision in PRECISION)) throw new Error(`hip backend: unknown precision "${precision}" (fp64 | fp32)`);
   if (!(boundary in BOUNDARY)) throw new Error(`hip backend: unknown boundary "${boundary}" (reference | matlab)`);
+  if (PRECISION[precision] === PRECISION.fp32 && boundary === 'reference' && !warnedFp32Reference) {
+    warnedFp32Reference = true;
+    console.warn(FP32_REFERENCE_WARNING);
+  }
   let tuning = pick(o.tuning, env.FDTD_HIP_TUNING || 'auto');
   if (typeof tuning === 'string' && tuning !== 'auto' && tuning !== 'default') {
     const [p, c, s, z] = tuning.split(',').map(Number);

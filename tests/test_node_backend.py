@@ -123,6 +123,21 @@ def test_backend_option_validation(tmp_path):
                                       "fuseCpml": 0}
     assert out["tuned"]["precision"] == "fp32" and out["tuned"]["batchSize"] == 7
     assert all(out[k] == "rejected" for k in out if k.startswith("{"))
+    # FP32 under the reference boundary (the default) warns once per process.
+    warnings = [ln for ln in res.stderr.splitlines() if "precision 'fp32' with boundary 'reference'" in ln]
+    assert len(warnings) == 1, res.stderr
+
+
+def test_fp32_matlab_boundary_does_not_warn(tmp_path):
+    script = tmp_path / "opts.mjs"
+    script.write_text(
+        "import { resolveOptions } from '%s';\n"
+        "console.log(JSON.stringify(resolveOptions({ precision: 'fp32', boundary: 'matlab' })));\n"
+        % os.path.join(JS, "backends", "hipBackend.mjs"))
+    res = node([str(script)], env={"FDTD_HIP_PRECISION": "", "FDTD_HIP_TUNING": "", "FDTD_HIP_BOUNDARY": ""})
+    assert res.returncode == 0, res.stderr
+    assert json.loads(res.stdout)["boundary"] == "matlab"
+    assert "boundary 'reference'" not in res.stderr
 
 
 # ------------------------------------------------------------------------------ GPU
