@@ -120,6 +120,7 @@ struct FdtdHipContext {
     void *prof = nullptr;       // b_e, a_e, b_m, a_m (T, nc each)
     void *psi[4] = {};
     void *cpsi[4] = {};
+    int cpsi_mask[4] = {-1, -1, -1, -1};   // 0: every CPsi element of the slot is equal
     long long n[4] = {};
   } face[6];
 
@@ -248,7 +249,11 @@ FaceSet<T> make_faces(const FdtdHipContext *c) {
     const T *prof = (const T *)h.prof;
     F.b_e = prof; F.a_e = prof ? prof + h.nc : nullptr;
     F.b_m = prof ? prof + 2 * h.nc : nullptr; F.a_m = prof ? prof + 3 * h.nc : nullptr;
-    for (int s = 0; s < 4; ++s) { F.psi[s] = (T *)h.psi[s]; F.cpsi[s] = (const T *)h.cpsi[s]; }
+    for (int s = 0; s < 4; ++s) {
+      F.psi[s] = (T *)h.psi[s];
+      F.cpsi[s] = (const T *)h.cpsi[s];
+      F.cmask[s] = h.cpsi_mask[s];
+    }
   }
   return fs;
 }
@@ -283,6 +288,15 @@ int dev_alloc(FdtdHipContext *c, void **p, size_t bytes) {
   HIP_TRY(hipMalloc(p, bytes ? bytes : 16));
   HIP_TRY(hipMemsetAsync(*p, 0, bytes ? bytes : 16, c->stream));
   return FDTD_HIP_OK;
+}
+
+// Whether every element of v has the same bits as the first (false when v is empty).
+template <typename T>
+bool all_equal_bits(const std::vector<T> &v) {
+  if (v.empty()) return false;
+  for (const T &x : v)
+    if (std::memcmp(&x, &v[0], sizeof(T)) != 0) return false;
+  return true;
 }
 
 template <typename T>
@@ -986,11 +1000,15 @@ int fdtd_hip_upload_cpml(FdtdHipContext *c, int faceId, int nc, const double *b_
   for (int s = 0; s < 4; ++s) {
     F.n[s] = nn[s];
     if (dev_alloc(c, &F.psi[s], nn[s] * c->esz) || dev_alloc(c, &F.cpsi[s], nn[s] * c->esz)) return FDTD_HIP_ERR;
+    // The whole array is uploaded either way; a slot whose elements are all equal (bit
+    // for bit, in the context's precision) is read at element 0 only.
     if (c->fp32()) {
       std::vector<float> v(cp[s], cp[s] + nn[s]);
+      F.cpsi_mask[s] = all_equal_bits(v) ? 0 : -1;
       if (upload_vec(c, F.cpsi[s], v)) return FDTD_HIP_ERR;
     } else {
       std::vector<double> v(cp[s], cp[s] + nn[s]);
+      F.cpsi_mask[s] = all_equal_bits(v) ? 0 : -1;
       if (upload_vec(c, F.cpsi[s], v)) return FDTD_HIP_ERR;
     }
   }

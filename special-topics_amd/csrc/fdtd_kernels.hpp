@@ -171,9 +171,14 @@ __device__ __forceinline__ void coeffs(const CoefTables &ct, const TableCache<T,
   }
 }
 
+// One CPML face.  CPsi of slot s is read at element q & cmask[s]: cmask is 0 when every
+// element of the slot holds the same value (a face whose slab is one material, as CPML
+// slabs usually are), so all reads hit element 0 in cache instead of streaming the
+// array; it is -1 otherwise.
 template <typename T>
 struct Face {
   int nc, m_start, e_start;
+  int cmask[4];
   const T *b_e, *a_e, *b_m, *a_m;
   T *psi[4];
   const T *cpsi[4];
@@ -213,7 +218,7 @@ __device__ __forceinline__ T cpml_step(const FaceSet<T> &fs, int s, int ci, T va
   const T a = MAG ? F.a_m[ci] : F.a_e[ci];
   const T ps = b * F.psi[SLOT][q] + a * diff;
   F.psi[SLOT][q] = ps;
-  return val + F.cpsi[SLOT][q] * ps;
+  return val + F.cpsi[SLOT][q & F.cmask[SLOT]] * ps;
 }
 
 // One component's CPML terms on the value v: faces of axis A1 (n then p) then of axis
@@ -273,8 +278,9 @@ __device__ __forceinline__ void fused_axis_terms(const FaceSet<T> &fs, int i, in
   const int qp0 = psi_offset<A>(i, j, k0, ci0, F.nc, p1, p2), qq0 = psi_offset<A>(i, j, k0, ci0, F.nc, q1, q2);
   T *__restrict__ psP = F.psi[SP] + qp0;
   T *__restrict__ psQ = F.psi[SQ] + qq0;
-  const T *__restrict__ cpP = F.cpsi[SP] + qp0;
-  const T *__restrict__ cpQ = F.cpsi[SQ] + qq0;
+  const int mp = F.cmask[SP], mq = F.cmask[SQ];
+  const T *__restrict__ cpP = F.cpsi[SP] + (qp0 & mp);
+  const T *__restrict__ cpQ = F.cpsi[SQ] + (qq0 & mq);
   const T *__restrict__ bb = (MAG ? F.b_m : F.b_e) + ci0;
   const T *__restrict__ aa = (MAG ? F.a_m : F.a_e) + ci0;
   constexpr int NB = A == 2 ? W : 1;                       // profile entries per lane
@@ -286,8 +292,8 @@ __device__ __forceinline__ void fused_axis_terms(const FaceSet<T> &fs, int i, in
       b[A == 2 ? e : 0] = bb[A == 2 ? e : 0];
       a[A == 2 ? e : 0] = aa[A == 2 ? e : 0];
     }
-    if (okp >> e & 1u) { psp[e] = psP[e]; cpp[e] = cpP[e]; }
-    if (okq >> e & 1u) { psq[e] = psQ[e]; cpq[e] = cpQ[e]; }
+    if (okp >> e & 1u) { psp[e] = psP[e]; cpp[e] = cpP[e & mp]; }
+    if (okq >> e & 1u) { psq[e] = psQ[e]; cpq[e] = cpQ[e & mq]; }
   }
 #pragma unroll
   for (int e = 0; e < W; ++e) {
@@ -361,7 +367,7 @@ __device__ __forceinline__ void cpml_load_face(const FaceSet<T> &fs, int s, int 
   const int ci = (A == 0 ? i : (A == 1 ? j : k)) - (MAG ? F.m_start : F.e_start);
   const int q = psi_offset<A>(i, j, k, ci, F.nc, d1, d2);
   ps = F.psi[SLOT][q];
-  cp = F.cpsi[SLOT][q];
+  cp = F.cpsi[SLOT][q & F.cmask[SLOT]];
   if (cpml_stage_profile<T>()) {
     b = MAG ? F.b_m[ci] : F.b_e[ci];
     a = MAG ? F.a_m[ci] : F.a_e[ci];
