@@ -86,10 +86,14 @@ def cpu_baseline(problem64, steps: int, threads: int, target_s: float = 15.0):
     return nx * ny * nz * steps / dt / 1e6, dt, steps
 
 
-def traffic_table(workload: str, precision: str) -> dict:
+def traffic_table(workload: str, precision: str, placement: int = 0) -> dict:
     """Per-kernel HBM bytes per launch from the committed rocprofv3 FETCH_SIZE /
-    WRITE_SIZE summary of this workload (calibrated against a 1 GiB copy); {} if none."""
-    path = os.path.join(REPO, "profiles", f"{workload}-{precision}-traffic.json")
+    WRITE_SIZE summary of this workload (calibrated against a 1 GiB copy); {} if none.
+    The CPML pass kernels keep their names across placements but not their tile lists,
+    so each placement other than the separate pass has a file of its own
+    (<workload>-<precision>-cpml<placement>-traffic.json)."""
+    tag = "" if placement == 0 else f"-cpml{placement}"
+    path = os.path.join(REPO, "profiles", f"{workload}-{precision}{tag}-traffic.json")
     try:
         with open(path) as f:
             return {k: int(v["bytes"]) for k, v in json.load(f)["kernels"].items()}
@@ -360,7 +364,7 @@ def stencil_study(workload, precision, steps, warmup, timing_steps, workdir, ntf
             return f"fdtd::k_update_{phase}_split<{ctype}, {coef_mode}, {w}>"
         return f"fdtd::k_update_{phase}<{ctype}, {coef_mode}, {w}, {tuning[3]}, {nt}>"
     kname = stencil_name("h" if dom == 0 else "e")
-    table = traffic_table(workload, precision)
+    table = traffic_table(workload, precision, tuning[3])
     traffic = table.get(kname)
     step_s = elapsed / steps
     # Whole step, counted: the two stencils, the two CPML passes and the small E-tail
