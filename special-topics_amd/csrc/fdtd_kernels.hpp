@@ -71,6 +71,52 @@ __device__ __forceinline__ void load_tables(const Grid<T> &g, int first, TableCa
   __syncthreads();
 }
 
+// The same fill in two halves, for stencils on grids that stay in the on-die caches:
+// table_issue loads a thread's share of the three tables into registers, the stencil
+// then issues its first plane's loads, and table_commit stores the share to LDS and
+// waits at the workgroup barrier.  The barrier's fences cover LDS only, so the plane's
+// loads stay in flight across it and the table's round trip overlaps theirs.
+template <typename T, int CM>
+struct TablePending {
+  T v[CM == COEF_U8 ? 3 * 3 : 1];
+};
+
+template <typename T, int CM>
+__device__ __forceinline__ void table_issue(const Grid<T> &g, int first, TablePending<T, CM> &tp) {
+  if (CM != COEF_U8) return;
+#pragma unroll
+  for (int comp = 0; comp < 3; ++comp) {
+    const T *tab = (const T *)g.ct.tab[first + comp];
+    const int n3 = 3 * g.ct.n[first + comp];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      const int r = (int)threadIdx.x + 256 * q;
+      tp.v[comp * 3 + q] = r < n3 ? tab[r] : T(0);
+    }
+  }
+}
+
+template <typename T, int CM>
+__device__ __forceinline__ void table_commit(TableCache<T, CM> &tc, const TablePending<T, CM> &tp) {
+  if (CM != COEF_U8) return;
+#pragma unroll
+  for (int comp = 0; comp < 3; ++comp)
+#pragma unroll
+    for (int q = 0; q < 3; ++q) tc.v[comp * 3 * U8_TABLE + (int)threadIdx.x + 256 * q] = tp.v[comp * 3 + q];
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// Whether a stencil kernel overlaps its table fill with its first plane's loads: 8-bit
+// tables in LDS, plain (cache-resident) accesses, and a placement whose stencil skips no
+// cell (the separate pass or the x/y faces fused), so that every thread of the workgroup
+// reaches the barrier at the same point of its first plane.
+template <int CM, int FC, bool NT>
+constexpr bool early_tables() {
+  return CM == COEF_U8 && !NT && (FC == CPML_SEPARATE || FC == CPML_FUSE_XY);
+}
+
 // W consecutive cells of one row, loaded and stored as one W*sizeof(T)-byte access.
 template <typename T, int W>
 struct alignas(sizeof(T) * W) Vec {
@@ -128,6 +174,50 @@ __device__ __forceinline__ Vec<T, W> ldv_stream(const T *__restrict__ p) {
   return r;
 }
 
+// The coefficient indices of W cells as loaded (8- or 16-bit storage), decoded only in
+// coeff_lookup, so that an index load issued ahead of its use is not waited for at once:
+// the 8-bit indices of the W cells packed in raw[0] (byte e = cell e), or one 16-bit
+// index per element.
+template <int CM, int W>
+struct CoefIdx {
+  uint32_t raw[CM == COEF_U16 ? W : 1];
+};
+
+// Index loads of component `first + local` for the W cells starting at offset c:
+// streaming (NT) or plain, as the fields.
+template <int CM, int W, bool NT = true>
+__device__ __forceinline__ CoefIdx<CM, W> coeff_index(const CoefTables &ct, int first, int local, int c) {
+  auto ld = [](auto *q) { return NT ? __builtin_nontemporal_load(q) : *q; };
+  CoefIdx<CM, W> r;
+  if (CM == COEF_U8) {
+    const uint8_t *ip = (const uint8_t *)ct.idx[first + local] + c;
+    if (W == 4) r.raw[0] = ld((const uint32_t *)ip);
+    else if (W == 2) r.raw[0] = ld((const uint16_t *)ip);
+    else r.raw[0] = ld(ip);
+  } else {
+    const uint16_t *ip = (const uint16_t *)ct.idx[first + local] + c;
+#pragma unroll
+    for (int e = 0; e < (CM == COEF_U16 ? W : 1); ++e) r.raw[e] = ld(ip + e);
+  }
+  return r;
+}
+
+// The (self, curl1, curl2) coefficients of component `first + local` at the W indices:
+// from the LDS copy (8-bit) or the table in global memory (16-bit).
+template <typename T, int CM, int W>
+__device__ __forceinline__ void coeff_lookup(const CoefTables &ct, const TableCache<T, CM> &tc, int first, int local,
+                                             const CoefIdx<CM, W> &ix, Vec<T, W> &c0, Vec<T, W> &c1,
+                                             Vec<T, W> &c2) {
+  const T *t = CM == COEF_U8 ? tc.v + local * 3 * U8_TABLE : (const T *)ct.tab[first + local];
+#pragma unroll
+  for (int e = 0; e < W; ++e) {
+    const int id = 3 * (int)(CM == COEF_U8 ? (ix.raw[0] >> (8 * e)) & 0xff : ix.raw[CM == COEF_U16 ? e : 0]);
+    c0[e] = t[id];
+    c1[e] = t[id + 1];
+    c2[e] = t[id + 2];
+  }
+}
+
 // The (self, curl1, curl2) coefficients of component `first + local` for the W cells
 // starting at offset c.
 template <typename T, int CM, int W, bool NT = true>
@@ -140,35 +230,7 @@ __device__ __forceinline__ void coeffs(const CoefTables &ct, const TableCache<T,
     c2 = ldv_stream<T, W, NT>(a2 + c);
     return;
   }
-  // Index loads: streaming (NT) or plain, as the fields.
-  auto ld = [](auto *q) { return NT ? __builtin_nontemporal_load(q) : *q; };
-  int id[W];
-  if (CM == COEF_U8) {
-    const uint8_t *ip = (const uint8_t *)ct.idx[first + local] + c;
-    if (W == 4) {
-      const uint32_t x = ld((const uint32_t *)ip);
-#pragma unroll
-      for (int e = 0; e < W; ++e) id[e] = 3 * (int)((x >> (8 * e)) & 0xff);
-    } else if (W == 2) {
-      const uint16_t x = ld((const uint16_t *)ip);
-#pragma unroll
-      for (int e = 0; e < W; ++e) id[e] = 3 * (int)((x >> (8 * e)) & 0xff);
-    } else {
-#pragma unroll
-      for (int e = 0; e < W; ++e) id[e] = 3 * (int)ld(ip + e);
-    }
-  } else {
-    const uint16_t *ip = (const uint16_t *)ct.idx[first + local] + c;
-#pragma unroll
-    for (int e = 0; e < W; ++e) id[e] = 3 * (int)ld(ip + e);
-  }
-  const T *t = CM == COEF_U8 ? tc.v + local * 3 * U8_TABLE : (const T *)ct.tab[first + local];
-#pragma unroll
-  for (int e = 0; e < W; ++e) {
-    c0[e] = t[id[e]];
-    c1[e] = t[id[e] + 1];
-    c2[e] = t[id[e] + 2];
-  }
+  coeff_lookup<T, CM, W>(ct, tc, first, local, coeff_index<CM, W, NT>(ct, first, local, c), c0, c1, c2);
 }
 
 // One CPML face.  CPsi of slot s is read at element q & cmask[s]: cmask is 0 when every
@@ -493,11 +555,16 @@ struct Fields {
 // gives the same result as the separate k_cpml pass.  Every array has a zero guard
 // plane before plane 0 and after plane nx, so all neighbour loads are unconditional;
 // elements outside a component's Yee shape are written back unchanged.
-template <typename T, int CM, int W, int FC, bool NT = true>
+// EARLY (early_tables): the kernel has issued its table loads into *tp and the LDS table
+// is filled here, by table_commit, once the first plane's loads are out.  Every thread of
+// the workgroup calls in, those past the plane's last row (active false) with j = k0 = 0
+// so that their loads stay in bounds; they leave after the barrier.
+template <typename T, int CM, int W, int FC, bool NT = true, bool EARLY = false>
 __device__ __forceinline__ void update_h_cells(
     T *__restrict__ Hx, T *__restrict__ Hy, T *__restrict__ Hz,
     const T *__restrict__ Ex, const T *__restrict__ Ey, const T *__restrict__ Ez,
-    const Grid<T> &g, const TableCache<T, CM> &tc, int j, int k0, int i0, int chunk, const FaceSet<T> &fs) {
+    const Grid<T> &g, TableCache<T, CM> &tc, int j, int k0, int i0, int chunk, const FaceSet<T> &fs,
+    const TablePending<T, CM> *tp = nullptr, bool active = true) {
 #pragma clang fp contract(off)
   typedef Vec<T, W> V;
   const int nx = g.nx, ny = g.ny, nz = g.nz, SJ = g.SJ, SI = g.SI;
@@ -526,9 +593,21 @@ __device__ __forceinline__ void update_h_cells(
     const T ey_w = Ey[c + W], ex_w = Ex[c + W];           // first cell past the vector
     V hx = ldv_stream<T, W, NT>(Hx + c), hy = ldv_stream<T, W, NT>(Hy + c), hz = ldv_stream<T, W, NT>(Hz + c);
     V c0, c1, c2, c3, c4, c5, c6, c7, c8;
-    coeffs<T, CM, W, NT>(g.ct, tc, 0, 0, c, g.C[CHXH], g.C[CHXEY], g.C[CHXEZ], c0, c1, c2);
-    coeffs<T, CM, W, NT>(g.ct, tc, 0, 1, c, g.C[CHYH], g.C[CHYEZ], g.C[CHYEX], c3, c4, c5);
-    coeffs<T, CM, W, NT>(g.ct, tc, 0, 2, c, g.C[CHZH], g.C[CHZEX], g.C[CHZEY], c6, c7, c8);
+    if (EARLY) {
+      const CoefIdx<CM, W> x0 = coeff_index<CM, W, NT>(g.ct, 0, 0, c), x1 = coeff_index<CM, W, NT>(g.ct, 0, 1, c),
+                           x2 = coeff_index<CM, W, NT>(g.ct, 0, 2, c);
+      if (i == i0) {
+        table_commit<T, CM>(tc, *tp);
+        if (!active) return;
+      }
+      coeff_lookup<T, CM, W>(g.ct, tc, 0, 0, x0, c0, c1, c2);
+      coeff_lookup<T, CM, W>(g.ct, tc, 0, 1, x1, c3, c4, c5);
+      coeff_lookup<T, CM, W>(g.ct, tc, 0, 2, x2, c6, c7, c8);
+    } else {
+      coeffs<T, CM, W, NT>(g.ct, tc, 0, 0, c, g.C[CHXH], g.C[CHXEY], g.C[CHXEZ], c0, c1, c2);
+      coeffs<T, CM, W, NT>(g.ct, tc, 0, 1, c, g.C[CHYH], g.C[CHYEZ], g.C[CHYEX], c3, c4, c5);
+      coeffs<T, CM, W, NT>(g.ct, tc, 0, 2, c, g.C[CHZH], g.C[CHZEX], g.C[CHZEY], c6, c7, c8);
+    }
     // The six E differences of the curls; the CPML terms use the same ones (E does not
     // change during the H phase).
     V dy_hx, dz_hx, dx_hy, dz_hy, dx_hz, dy_hz;
@@ -599,7 +678,7 @@ template <typename T, int CM, int W, int FC, bool NT = true>
 __device__ __forceinline__ void update_h_body(
     T *__restrict__ Hx, T *__restrict__ Hy, T *__restrict__ Hz,
     const T *__restrict__ Ex, const T *__restrict__ Ey, const T *__restrict__ Ez,
-    const Grid<T> &g, const MarchGeom &m, const TableCache<T, CM> &tc, int col, int ch, const FaceSet<T> &fs) {
+    const Grid<T> &g, const MarchGeom &m, TableCache<T, CM> &tc, int col, int ch, const FaceSet<T> &fs) {
   const int p = (col * 256 + (int)threadIdx.x) * W;
   const int j = p / g.SJ;
   if (j > g.ny) return;                                    // plane padding
@@ -611,8 +690,18 @@ __global__ void __launch_bounds__(256) k_update_h(Grid<T> g, MarchGeom m, FaceSe
   __shared__ TableCache<T, CM> tc;
   int col, ch;
   if (!march_tile(m, col, ch)) return;
-  load_tables<T, CM>(g, 0, tc);
-  update_h_body<T, CM, W, FC, NT>(g.Hx, g.Hy, g.Hz, g.Ex, g.Ey, g.Ez, g, m, tc, col, ch, fs);
+  if constexpr (early_tables<CM, FC, NT>()) {
+    TablePending<T, CM> tp;
+    table_issue<T, CM>(g, 0, tp);
+    const int p = (col * 256 + (int)threadIdx.x) * W;
+    const int j = p / g.SJ;
+    const bool active = j <= g.ny;                         // else plane padding
+    update_h_cells<T, CM, W, FC, NT, true>(g.Hx, g.Hy, g.Hz, g.Ex, g.Ey, g.Ez, g, tc, active ? j : 0,
+                                           active ? p - j * g.SJ : 0, ch * m.chunk, m.chunk, fs, &tp, active);
+  } else {
+    load_tables<T, CM>(g, 0, tc);
+    update_h_body<T, CM, W, FC, NT>(g.Hx, g.Hy, g.Hz, g.Ex, g.Ey, g.Ez, g, m, tc, col, ch, fs);
+  }
 }
 
 // Bulk update of the i = nx wall's Ey/Ez at (j, k) under the reference boundary (see
@@ -668,12 +757,13 @@ __device__ __forceinline__ void e_wall_terms(const Grid<T> &g, const FaceSet<T> 
 // element j*nz+k of Hz.  With ghost == 0 the wall is left to the CPML terms.  With
 // every face fused (FC == CPML_FUSE_ALL) the same thread then applies the wall's CPML
 // terms, which read only H.  Elements the bulk does not update keep their value.
-template <typename T, int CM, int W, int FC, bool NT = true>
+// EARLY, tp and active: as for update_h_cells.
+template <typename T, int CM, int W, int FC, bool NT = true, bool EARLY = false>
 __device__ __forceinline__ void update_e_cells(
     T *__restrict__ Ex, T *__restrict__ Ey, T *__restrict__ Ez,
     const T *__restrict__ Hx, const T *__restrict__ Hy, const T *__restrict__ Hz,
-    const Grid<T> &g, const TableCache<T, CM> &tc, int j, int k0, int i0, int chunk, int ghost,
-    const FaceSet<T> &fs) {
+    const Grid<T> &g, TableCache<T, CM> &tc, int j, int k0, int i0, int chunk, int ghost,
+    const FaceSet<T> &fs, const TablePending<T, CM> *tp = nullptr, bool active = true) {
 #pragma clang fp contract(off)
   typedef Vec<T, W> V;
   const int nx = g.nx, ny = g.ny, nz = g.nz, SJ = g.SJ, SI = g.SI;
@@ -705,9 +795,21 @@ __device__ __forceinline__ void update_e_cells(
     const T hy_w = Hy[c - 1], hx_w = Hx[c - 1];            // last cell before the vector
     V ex = ldv_stream<T, W, NT>(Ex + c), ey = ldv_stream<T, W, NT>(Ey + c), ez = ldv_stream<T, W, NT>(Ez + c);
     V c0, c1, c2, c3, c4, c5, c6, c7, c8;
-    coeffs<T, CM, W, NT>(g.ct, tc, 3, 0, c, g.C[CEXE], g.C[CEXHZ], g.C[CEXHY], c0, c1, c2);
-    coeffs<T, CM, W, NT>(g.ct, tc, 3, 1, c, g.C[CEYE], g.C[CEYHX], g.C[CEYHZ], c3, c4, c5);
-    coeffs<T, CM, W, NT>(g.ct, tc, 3, 2, c, g.C[CEZE], g.C[CEZHY], g.C[CEZHX], c6, c7, c8);
+    if (EARLY) {
+      const CoefIdx<CM, W> x0 = coeff_index<CM, W, NT>(g.ct, 3, 0, c), x1 = coeff_index<CM, W, NT>(g.ct, 3, 1, c),
+                           x2 = coeff_index<CM, W, NT>(g.ct, 3, 2, c);
+      if (i == i0) {
+        table_commit<T, CM>(tc, *tp);
+        if (!active) return;
+      }
+      coeff_lookup<T, CM, W>(g.ct, tc, 3, 0, x0, c0, c1, c2);
+      coeff_lookup<T, CM, W>(g.ct, tc, 3, 1, x1, c3, c4, c5);
+      coeff_lookup<T, CM, W>(g.ct, tc, 3, 2, x2, c6, c7, c8);
+    } else {
+      coeffs<T, CM, W, NT>(g.ct, tc, 3, 0, c, g.C[CEXE], g.C[CEXHZ], g.C[CEXHY], c0, c1, c2);
+      coeffs<T, CM, W, NT>(g.ct, tc, 3, 1, c, g.C[CEYE], g.C[CEYHX], g.C[CEYHZ], c3, c4, c5);
+      coeffs<T, CM, W, NT>(g.ct, tc, 3, 2, c, g.C[CEZE], g.C[CEZHY], g.C[CEZHX], c6, c7, c8);
+    }
     // The six H differences of the curls, shared with the CPML terms (H does not change
     // during the E phase).
     V dy_ex, dz_ex, dz_ey, dx_ey, dx_ez, dy_ez;
@@ -802,7 +904,7 @@ template <typename T, int CM, int W, int FC, bool NT = true>
 __device__ __forceinline__ void update_e_body(
     T *__restrict__ Ex, T *__restrict__ Ey, T *__restrict__ Ez,
     const T *__restrict__ Hx, const T *__restrict__ Hy, const T *__restrict__ Hz,
-    const Grid<T> &g, const MarchGeom &m, const TableCache<T, CM> &tc, int col, int ch, int ghost,
+    const Grid<T> &g, const MarchGeom &m, TableCache<T, CM> &tc, int col, int ch, int ghost,
     const FaceSet<T> &fs) {
   const int p = (col * 256 + (int)threadIdx.x) * W;
   const int j = p / g.SJ;
@@ -815,8 +917,19 @@ __global__ void __launch_bounds__(256) k_update_e(Grid<T> g, MarchGeom m, int gh
   __shared__ TableCache<T, CM> tc;
   int col, ch;
   if (!march_tile(m, col, ch)) return;
-  load_tables<T, CM>(g, 3, tc);
-  update_e_body<T, CM, W, FC, NT>(g.Ex, g.Ey, g.Ez, g.Hx, g.Hy, g.Hz, g, m, tc, col, ch, ghost, fs);
+  if constexpr (early_tables<CM, FC, NT>()) {
+    TablePending<T, CM> tp;
+    table_issue<T, CM>(g, 3, tp);
+    const int p = (col * 256 + (int)threadIdx.x) * W;
+    const int j = p / g.SJ;
+    const bool active = j <= g.ny;                         // else plane padding
+    update_e_cells<T, CM, W, FC, NT, true>(g.Ex, g.Ey, g.Ez, g.Hx, g.Hy, g.Hz, g, tc, active ? j : 0,
+                                           active ? p - j * g.SJ : 0, ch * m.chunk, m.chunk, ghost, fs, &tp,
+                                           active);
+  } else {
+    load_tables<T, CM>(g, 3, tc);
+    update_e_body<T, CM, W, FC, NT>(g.Ex, g.Ey, g.Ez, g.Hx, g.Hy, g.Hz, g, m, tc, col, ch, ghost, fs);
+  }
 }
 
 // CPML_SPLIT_Z: one launch of the bulk update in which workgroups [0, z.ntiles) march
