@@ -93,8 +93,19 @@ __global__ void __launch_bounds__(256) k_multi_update_h(const VariantDesc<T> *__
   const int col = local % d.mh.ncol, ch = local / d.mh.ncol;
   const Grid<T> g = global_grid(d.g);
   const FaceSet<T> fs = global_faces(d.fs);
-  load_tables<T, CM>(g, 0, tc);
-  update_h_body<T, CM, W, FC>(g.Hx, g.Hy, g.Hz, g.Ex, g.Ey, g.Ez, g, d.mh, tc, col, ch, fs);
+  if constexpr (early_tables<CM, FC>()) {
+    TablePending<T, CM> tp;
+    table_issue<T, CM>(g, 0, tp);
+    const int p = (col * 256 + (int)threadIdx.x) * W;
+    const int j = p / g.SJ;
+    const bool active = j <= g.ny;                         // else plane padding
+    update_h_cells<T, CM, W, FC, true, true>(g.Hx, g.Hy, g.Hz, g.Ex, g.Ey, g.Ez, g, tc, active ? j : 0,
+                                             active ? p - j * g.SJ : 0, ch * d.mh.chunk, d.mh.chunk, fs, &tp,
+                                             active);
+  } else {
+    load_tables<T, CM>(g, 0, tc);
+    update_h_body<T, CM, W, FC>(g.Hx, g.Hy, g.Hz, g.Ex, g.Ey, g.Ez, g, d.mh, tc, col, ch, fs);
+  }
 }
 
 template <typename T, int CM, int W, int FC>
@@ -108,8 +119,19 @@ __global__ void __launch_bounds__(256) k_multi_update_e(const VariantDesc<T> *__
   const int col = local % d.me.ncol, ch = local / d.me.ncol;
   const Grid<T> g = global_grid(d.g);
   const FaceSet<T> fs = global_faces(d.fs);
-  load_tables<T, CM>(g, 3, tc);
-  update_e_body<T, CM, W, FC>(g.Ex, g.Ey, g.Ez, g.Hx, g.Hy, g.Hz, g, d.me, tc, col, ch, d.ghost, fs);
+  if constexpr (early_tables<CM, FC>()) {
+    TablePending<T, CM> tp;
+    table_issue<T, CM>(g, 3, tp);
+    const int p = (col * 256 + (int)threadIdx.x) * W;
+    const int j = p / g.SJ;
+    const bool active = j <= g.ny;                         // else plane padding
+    update_e_cells<T, CM, W, FC, true, true>(g.Ex, g.Ey, g.Ez, g.Hx, g.Hy, g.Hz, g, tc, active ? j : 0,
+                                             active ? p - j * g.SJ : 0, ch * d.me.chunk, d.me.chunk, d.ghost, fs,
+                                             &tp, active);
+  } else {
+    load_tables<T, CM>(g, 3, tc);
+    update_e_body<T, CM, W, FC>(g.Ex, g.Ey, g.Ez, g.Hx, g.Hy, g.Hz, g, d.me, tc, col, ch, d.ghost, fs);
+  }
 }
 
 // The CPML pass: one workgroup per tile of the variant's tile list.
