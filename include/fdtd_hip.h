@@ -314,7 +314,9 @@ int fdtd_hip_multi_set_graphs(FdtdHipMulti *m, int enable);
 int fdtd_hip_multi_get_tuning(FdtdHipMulti *m, FdtdHipTuning *tuning);
 /* Time the batched launches for every cells-per-thread / planes / storage / CPML
  * placement choice over a few steps, keep the fastest, and reset every variant (before
- * its first step only). */
+ * its first step only).  Per-cell coefficient arrays are timed only when some variant's
+ * tables need wider than 8-bit indices, and a candidate whose warm-up step already takes
+ * 1.5 times the best timed step is not timed further. */
 int fdtd_hip_multi_autotune(FdtdHipMulti *m, int steps_per_candidate, FdtdHipTuning *chosen);
 void fdtd_hip_multi_destroy(FdtdHipMulti *m);
 

@@ -2008,14 +2008,21 @@ int multi_steps(FdtdHipMulti *m, int ts, int count, bool graphs) {
   return FDTD_HIP_OK;
 }
 
+// Steps [ts, ts+count) of every variant with the variants' streams already lent
+// (multi_borrow): descriptors for the current tuning, then the launches.
+int multi_run_lent(FdtdHipMulti *m, int ts, int count, bool graphs) {
+  const int rc = m->fp32 ? multi_prepare<float>(m) : multi_prepare<double>(m);
+  if (rc) return rc;
+  return m->fp32 ? multi_steps<float>(m, ts, count, graphs) : multi_steps<double>(m, ts, count, graphs);
+}
+
 int multi_run(FdtdHipMulti *m, int ts, int count, bool graphs) {
   for (FdtdHipContext *c : m->ctx)
     if (ts < 0 || count < 0 || ts + count > c->d.numberOfTimeSteps)
       return fail("multi_run: steps [%d,%d) outside [0,%d) of a variant", ts, ts + count, c->d.numberOfTimeSteps);
   HIP_TRY(hipSetDevice(m->device));
   if (multi_borrow(m)) return FDTD_HIP_ERR;
-  int rc = m->fp32 ? multi_prepare<float>(m) : multi_prepare<double>(m);
-  if (!rc) rc = m->fp32 ? multi_steps<float>(m, ts, count, graphs) : multi_steps<double>(m, ts, count, graphs);
+  const int rc = multi_run_lent(m, ts, count, graphs);
   const int rc2 = multi_return(m);
   if (rc || rc2) return FDTD_HIP_ERR;
   for (FdtdHipContext *c : m->ctx) c->steps_run += count;
@@ -2101,36 +2108,61 @@ int fdtd_hip_multi_autotune(FdtdHipMulti *m, int steps_per_candidate, FdtdHipTun
   if (steps_per_candidate < 1) steps_per_candidate = 2;
   int steps = steps_per_candidate + 1;
   for (FdtdHipContext *c : m->ctx) steps = std::min(steps, c->d.numberOfTimeSteps);
+  HIP_TRY(hipSetDevice(m->device));
+  // Per-cell coefficient arrays (storage 1) read 9 values per cell and phase where the
+  // 8-bit indices read 3 bytes, for the same arithmetic; they are timed only when some
+  // variant's tables need wider indices.
+  if (build_tables_threaded(m->ctx, true)) return FDTD_HIP_ERR;
+  bool all_u8 = true;
+  for (FdtdHipContext *c : m->ctx) all_u8 = all_u8 && c->coef_idx_bytes == 1;
   std::vector<FdtdHipTuning> cands;
-  for (int cs = 0; cs < 2; ++cs)
+  for (int cs = 0; cs < (all_u8 ? 1 : 2); ++cs)
     for (int w : {1, 2, 4}) {
       if (w == 4 && !m->fp32) continue;
       for (int pl : {1, 2, 4, 8})
         for (int fc : {CPML_SEPARATE, CPML_FUSE_Z, CPML_FUSE_ALL, CPML_FUSE_XY})
           cands.push_back(FdtdHipTuning{pl, w, cs, fc});
     }
-  hipEvent_t e0 = nullptr, e1 = nullptr;
-  HIP_TRY(hipSetDevice(m->device));
-  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
-    if (e0) (void)hipEventDestroy(e0);
-    return fail("multi_autotune: event creation failed");
-  }
-  double best = 1e300;
+  hipEvent_t ev[3] = {};
+  for (hipEvent_t &e : ev)
+    if (hipEventCreate(&e) != hipSuccess) {
+      for (hipEvent_t x : ev) if (x) (void)hipEventDestroy(x);
+      return fail("multi_autotune: event creation failed");
+    }
+  auto run_steps = [m](int ts, int n) {
+    return m->fp32 ? multi_steps<float>(m, ts, n, false) : multi_steps<double>(m, ts, n, false);
+  };
+  double best = 1e300;       // ms per timed step
   FdtdHipTuning best_t = m->tune;
-  int rc = FDTD_HIP_OK;
-  for (const auto &t : cands) {
+  // The variants' streams are lent once for the whole search.
+  int rc = multi_borrow(m);
+  for (size_t q = 0; !rc && q < cands.size(); ++q) {
+    const FdtdHipTuning &t = cands[q];
     m->tune = t;
-    // Step 0 warms up; steps 1.. are timed.
-    if ((rc = multi_run(m, 0, 1, false))) break;
-    if (hipEventRecord(e0, m->stream) != hipSuccess) { rc = fail("multi_autotune: event record failed"); break; }
-    if ((rc = multi_run(m, 1, steps - 1, false))) break;
-    float ms = 0.f;
-    if (hipEventRecord(e1, m->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
-        hipEventElapsedTime(&ms, e0, e1) != hipSuccess) { rc = fail("multi_autotune: event timing failed"); break; }
-    if (ms < best) { best = ms; best_t = t; }
+    if ((rc = m->fp32 ? multi_prepare<float>(m) : multi_prepare<double>(m))) break;
+    // Step 0 warms up (and is timed too: a candidate whose warm-up step takes more than
+    // 1.5 times the best timed step so far cannot win and is not timed further); steps
+    // 1.. are timed.
+    float warm = 0.f, ms = 0.f;
+    if (hipEventRecord(ev[0], m->stream) != hipSuccess || (rc = run_steps(0, 1)) ||
+        hipEventRecord(ev[1], m->stream) != hipSuccess || hipEventSynchronize(ev[1]) != hipSuccess ||
+        hipEventElapsedTime(&warm, ev[0], ev[1]) != hipSuccess) {
+      if (!rc) rc = fail("multi_autotune: event timing failed");
+      break;
+    }
+    double per = warm;
+    if (steps >= 2) {
+      if (warm > 1.5 * best) continue;
+      if (hipEventRecord(ev[0], m->stream) != hipSuccess) { rc = fail("multi_autotune: event record failed"); break; }
+      if ((rc = run_steps(1, steps - 1))) break;
+      if (hipEventRecord(ev[2], m->stream) != hipSuccess || hipEventSynchronize(ev[2]) != hipSuccess ||
+          hipEventElapsedTime(&ms, ev[0], ev[2]) != hipSuccess) { rc = fail("multi_autotune: event timing failed"); break; }
+      per = ms / (steps - 1);
+    }
+    if (per < best) { best = per; best_t = t; }
   }
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
+  if (multi_return(m) && !rc) rc = FDTD_HIP_ERR;
+  for (hipEvent_t e : ev) (void)hipEventDestroy(e);
   m->tune = best_t;
   for (FdtdHipContext *c : m->ctx) {
     c->steps_run = 0;
