@@ -206,6 +206,14 @@ int fdtd_hip_radiated_power(FdtdHipContext *ctx, double *out, int nFreq);
 int fdtd_hip_directivity(FdtdHipContext *ctx, const double *freqHz, const double *radPower, int nAngles,
                          const double *theta, const double *phi, double *dataTheta, double *dataPhi);
 
+/* fdtd_hip_directivity at the one NTFF frequency f (0 <= f < nFreq): freqHz is that
+ * frequency, radPower points to its radiated power (or is NULL), and dataTheta /
+ * dataPhi receive [nAngles] -- the bits of row f of fdtd_hip_directivity's result.
+ * No counterpart in the reference, whose calcDirectivity evaluates every frequency; for
+ * callers that need one frequency (a sweep's pattern at each variant's resonance). */
+int fdtd_hip_directivity_at(FdtdHipContext *ctx, int f, double freqHz, const double *radPower, int nAngles,
+                            const double *theta, const double *phi, double *dataTheta, double *dataPhi);
+
 /* fdtd_hip_directivity without any device allocation, for callers that cannot
  * allocate (code under HIP graph capture, tight serving loops, harnesses that forbid
  * hipMalloc).  No counterpart in the reference.  Same arguments and results as

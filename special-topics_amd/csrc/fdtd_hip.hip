@@ -1376,7 +1376,7 @@ int fdtd_hip_radiated_power(FdtdHipContext *c, double *out, int nFreq) {
   Scratch s;
   double *d = s.put<double>(c, nullptr, nFreq);
   if (!d) return fail("radiated_power: device allocation failed");
-  hipLaunchKernelGGL(k_ntff_power, dim3(nFreq), dim3(256), 0, c->stream, ntff_post_params(c), d);
+  hipLaunchKernelGGL(k_ntff_power, dim3(nFreq), dim3(256), 0, c->stream, ntff_post_params(c), d, 0);
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipMemcpyAsync(out, d, nFreq * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
@@ -1396,10 +1396,11 @@ struct DirLayout {
   }
 };
 
-static int directivity_impl(FdtdHipContext *c, const double *freqHz, const double *radPower, int nAngles,
-                            const double *theta, const double *phi, double *dataTheta, double *dataPhi,
-                            double *ws, const DirLayout &L) {
-  const int nf = c->nff;
+// Directivity at the nf NTFF frequencies f0 .. f0+nf-1 (freqHz and radPower hold nf
+// values each; radPower null: computed here).
+static int directivity_impl(FdtdHipContext *c, int f0, int nf, const double *freqHz, const double *radPower,
+                            int nAngles, const double *theta, const double *phi, double *dataTheta,
+                            double *dataPhi, double *ws, const DirLayout &L) {
   const size_t nout = (size_t)nf * nAngles;
   HIP_TRY(hipMemcpyAsync(ws + L.freq, freqHz, nf * sizeof(double), hipMemcpyHostToDevice, c->stream));
   HIP_TRY(hipMemcpyAsync(ws + L.theta, theta, nAngles * sizeof(double), hipMemcpyHostToDevice, c->stream));
@@ -1407,13 +1408,13 @@ static int directivity_impl(FdtdHipContext *c, const double *freqHz, const doubl
   if (radPower)
     HIP_TRY(hipMemcpyAsync(ws + L.prad, radPower, nf * sizeof(double), hipMemcpyHostToDevice, c->stream));
   else
-    hipLaunchKernelGGL(k_ntff_power, dim3(nf), dim3(256), 0, c->stream, ntff_post_params(c), ws + L.prad);
+    hipLaunchKernelGGL(k_ntff_power, dim3(nf), dim3(256), 0, c->stream, ntff_post_params(c), ws + L.prad, f0);
   const int nprep = std::max(nf, nAngles);
   hipLaunchKernelGGL(k_directivity_prep, dim3((nprep + 255) / 256), dim3(256), 0, c->stream, ws + L.freq,
                      ws + L.theta, ws + L.phi, nf, nAngles, ws + L.kw, ws + L.ang);
   constexpr int NA = 4;
   hipLaunchKernelGGL(k_ntff_directivity<NA>, dim3((nAngles + NA - 1) / NA, nf), dim3(256), 0, c->stream,
-                     ntff_post_params(c), ws + L.kw, ws + L.prad, ws + L.ang, nAngles, ws + L.dth, ws + L.dph);
+                     ntff_post_params(c), ws + L.kw, ws + L.prad, ws + L.ang, nAngles, ws + L.dth, ws + L.dph, f0);
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipMemcpyAsync(dataTheta, ws + L.dth, nout * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipMemcpyAsync(dataPhi, ws + L.dph, nout * sizeof(double), hipMemcpyDeviceToHost, c->stream));
@@ -1436,7 +1437,18 @@ int fdtd_hip_directivity(FdtdHipContext *c, const double *freqHz, const double *
   Scratch s;
   double *ws = s.put<double>(c, nullptr, (size_t)L.total);
   if (!ws) return fail("directivity: device allocation failed");
-  return directivity_impl(c, freqHz, radPower, nAngles, theta, phi, dataTheta, dataPhi, ws, L);
+  return directivity_impl(c, 0, c->nff, freqHz, radPower, nAngles, theta, phi, dataTheta, dataPhi, ws, L);
+}
+
+int fdtd_hip_directivity_at(FdtdHipContext *c, int f, double freqHz, const double *radPower, int nAngles,
+                            const double *theta, const double *phi, double *dataTheta, double *dataPhi) {
+  if (directivity_check(c, &freqHz, theta, phi, dataTheta, dataPhi, nAngles, "directivity_at")) return FDTD_HIP_ERR;
+  if (f < 0 || f >= c->nff) return fail("directivity_at: frequency %d outside [0,%d)", f, c->nff);
+  const DirLayout L(1, nAngles, nAngles);
+  Scratch s;
+  double *ws = s.put<double>(c, nullptr, (size_t)L.total);
+  if (!ws) return fail("directivity_at: device allocation failed");
+  return directivity_impl(c, f, 1, &freqHz, radPower, nAngles, theta, phi, dataTheta, dataPhi, ws, L);
 }
 
 int fdtd_hip_directivity_noalloc(FdtdHipContext *c, const double *freqHz, const double *radPower, int nAngles,
@@ -1450,7 +1462,7 @@ int fdtd_hip_directivity_noalloc(FdtdHipContext *c, const double *freqHz, const 
   double *ws = noalloc_workspace(c->device);
   if (!ws) return fail("directivity_noalloc: no workspace address for device %d", c->device);
   const DirLayout L(FDTD_HIP_NOALLOC_MAX_FREQ, FDTD_HIP_NOALLOC_MAX_ANGLES, FDTD_HIP_NOALLOC_MAX_OUTPUTS);
-  return directivity_impl(c, freqHz, radPower, nAngles, theta, phi, dataTheta, dataPhi, ws, L);
+  return directivity_impl(c, 0, c->nff, freqHz, radPower, nAngles, theta, phi, dataTheta, dataPhi, ws, L);
 }
 
 int fdtd_hip_set_ntff_mode(FdtdHipContext *c, int mode) {

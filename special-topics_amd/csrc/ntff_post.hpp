@@ -51,9 +51,10 @@ __device__ __forceinline__ void block_sum(double (&v)[V], double *lds /* 4*V */)
   }
 }
 
-__global__ void __launch_bounds__(256) k_ntff_power(NtffPost p, double *out) {
+// One block per frequency f0 + blockIdx.x of the accumulators; out[blockIdx.x].
+__global__ void __launch_bounds__(256) k_ntff_power(NtffPost p, double *out, int f0) {
   __shared__ double lds[4];
-  const int f = blockIdx.x;
+  const int f = f0 + (int)blockIdx.x;
   const long sx = (long)p.nj * p.nk, sy = (long)p.ni * p.nk, sz = (long)p.ni * p.nj;
   double acc[1] = {0.0};
   for (long q = threadIdx.x; q < sx + sy + sz; q += blockDim.x) {
@@ -77,7 +78,7 @@ __global__ void __launch_bounds__(256) k_ntff_power(NtffPost p, double *out) {
     }
   }
   block_sum<1>(acc, lds);
-  if (threadIdx.x == 0) out[f] = 0.5 * acc[0];
+  if (threadIdx.x == 0) out[blockIdx.x] = 0.5 * acc[0];
 }
 
 // Wave numbers k = 2 pi f / c and the angle table [nA][4] = sin th, cos th, sin ph,
@@ -95,13 +96,14 @@ __global__ void __launch_bounds__(256) k_directivity_prep(const double *freq, co
   }
 }
 
-// One block per (frequency, group of NA angles).
+// One block per (frequency, group of NA angles): frequency f0 + blockIdx.y of the
+// accumulators, entry blockIdx.y of kwave, prad and the outputs.
 template <int NA>
 __global__ void __launch_bounds__(256) k_ntff_directivity(NtffPost p, const double *kwave, const double *prad,
                                                           const double *ang /* [nA][4]: sin th, cos th, sin ph, cos ph */,
-                                                          int nA, double *dth, double *dph) {
+                                                          int nA, double *dth, double *dph, int f0) {
   __shared__ double lds[4 * 8 * NA];
-  const int f = blockIdx.y, a0 = blockIdx.x * NA;
+  const int f = blockIdx.y, fa = f0 + (int)blockIdx.y, a0 = blockIdx.x * NA;
   const long sx = (long)p.nj * p.nk, sy = (long)p.ni * p.nk, sz = (long)p.ni * p.nj;
   const double k = kwave[f];
   const double ci = 0.5 * (p.ui + p.li), cj = 0.5 * (p.uj + p.lj), ck = 0.5 * (p.uk + p.lk);
@@ -121,7 +123,7 @@ __global__ void __launch_bounds__(256) k_ntff_directivity(NtffPost p, const doub
     long cell;
     ntff_cell(p, q, axis, cell, r, c);
     const long fsz = axis == 0 ? sx : (axis == 1 ? sy : sz);
-    const long pos = (long)f * fsz + cell, im = fsz * p.nfreq;
+    const long pos = (long)fa * fsz + cell, im = fsz * p.nfreq;
     const double dA = axis == 0 ? p.dy * p.dz : (axis == 1 ? p.dx * p.dz : p.dx * p.dy);
     // Cell-centre offsets from the box centre along the two surface axes.
     const double ri = (p.li - 1) + r - ci + 0.5, rj = (p.lj - 1) + r - cj + 0.5;

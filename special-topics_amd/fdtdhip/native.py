@@ -56,7 +56,8 @@ EXPORTS = [
     "fdtd_hip_read_probes", "fdtd_hip_upload_farfield",
     "fdtd_hip_upload_port_dft", "fdtd_hip_run_batch", "fdtd_hip_run_batch_async", "fdtd_hip_read_traces",
     "fdtd_hip_query", "fdtd_hip_get_step_info", "fdtd_hip_read_field", "fdtd_hip_write_field",
-    "fdtd_hip_read_psi", "fdtd_hip_write_psi", "fdtd_hip_read_farfield", "fdtd_hip_read_port_dft", "fdtd_hip_radiated_power", "fdtd_hip_directivity", "fdtd_hip_directivity_noalloc", "fdtd_hip_get_counts",
+    "fdtd_hip_read_psi", "fdtd_hip_write_psi", "fdtd_hip_read_farfield", "fdtd_hip_read_port_dft", "fdtd_hip_radiated_power", "fdtd_hip_directivity", "fdtd_hip_directivity_at",
+    "fdtd_hip_directivity_noalloc", "fdtd_hip_get_counts",
     "fdtd_hip_set_ntff_mode", "fdtd_hip_get_ntff_mode", "fdtd_hip_set_graphs",
     "fdtd_hip_set_tuning", "fdtd_hip_get_tuning", "fdtd_hip_autotune",
     "fdtd_hip_get_coefficient_storage", "fdtd_hip_set_timing", "fdtd_hip_get_kernel_times", "fdtd_hip_synchronize", "fdtd_hip_destroy",
@@ -120,6 +121,7 @@ def lib():
         L.fdtd_hip_radiated_power.argtypes = [P, D, I]
         L.fdtd_hip_directivity.argtypes = [P, D, D, I, D, D, D, D]
         L.fdtd_hip_directivity_noalloc.argtypes = [P, D, D, I, D, D, D, D]
+        L.fdtd_hip_directivity_at.argtypes = [P, I, C.c_double, D, I, D, D, D, D]
         L.fdtd_hip_get_counts.argtypes = [P, C.POINTER(FdtdHipCounts)]
         L.fdtd_hip_set_ntff_mode.argtypes = [P, I]
         L.fdtd_hip_get_ntff_mode.argtypes = [P, IP]

@@ -215,6 +215,20 @@ class HipSim:
                      "directivity")
         return dt, dp
 
+    def directivity_at(self, f, theta, phi, rad_power=None):
+        """(dataTheta, dataPhi), each [nAngles], at NTFF frequency f only (row f of
+        directivity()).  rad_power: that frequency's radiated power, or None."""
+        th = np.ascontiguousarray(theta, dtype=np.float64)
+        ph = np.ascontiguousarray(phi, dtype=np.float64)
+        freqs = self.problem.header["ff"]["frequencies"]
+        fr = float(freqs[f]) if 0 <= f < len(freqs) else 0.0      # the library refuses f outside
+        dt = np.zeros(th.size)
+        dp = np.zeros(th.size)
+        rp = None if rad_power is None else _d(np.array([rad_power], dtype=np.float64))
+        native.check(self.L.fdtd_hip_directivity_at(self.h, int(f), fr, rp, th.size, _d(th), _d(ph), _d(dt), _d(dp)),
+                     "directivity_at")
+        return dt, dp
+
     def directivity_noalloc(self, theta, phi, rad_power=None):
         """As directivity(), through the entry point that allocates no device memory."""
         th = np.ascontiguousarray(theta, dtype=np.float64)

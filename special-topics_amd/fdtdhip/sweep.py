@@ -142,10 +142,16 @@ class Sweep:
             ff = np.asarray(p.header["ff"]["frequencies"])
             f = int(np.argmin(np.abs(ff - freqs[k])))
             prad = sim.radiated_power()
-            cuts = {}
-            for name, (th, ph) in self.cuts.items():
-                dt, dp = sim.directivity(th, ph, prad)
-                cuts[name] = (dt[f], dp[f])
+            # Every cut at the one frequency used, in one call.
+            names = list(self.cuts)
+            th = np.concatenate([self.cuts[n][0] for n in names])
+            ph = np.concatenate([self.cuts[n][1] for n in names])
+            dt, dp = sim.directivity_at(f, th, ph, prad[f])
+            cuts, a = {}, 0
+            for n in names:
+                m = len(self.cuts[n][0])
+                cuts[n] = (dt[a:a + m], dp[a:a + m])
+                a += m
             res.update({"farfield_hz": float(ff[f]), "prad_w": float(prad[f]), "cuts": cuts,
                         "peak_directivity_dbi": float(10 * np.log10(max(max(np.max(a), np.max(b))
                                                                         for a, b in cuts.values())))})
