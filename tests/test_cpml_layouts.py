@@ -1,7 +1,9 @@
 """The CPML pass over uneven slab layouts (needs the GPU): some faces absent (a plain
-wall there instead of CPML), unequal depths, in every CPML placement and both boundary
-modes.  FP64 must match the oracle bit for bit (traces, fields, every Psi); the batched
-launch must equal each single run.
+wall there instead of CPML), unequal depths, in every CPML placement, both boundary
+modes and one- and two-cell vectors.  FP64 must match the oracle bit for bit (traces,
+fields, every Psi); FP32 must give the same bits in every launch shape and placement
+(a vector of 2 or 4 cells may start before a slab or end past it, at either end); the
+batched launch must equal each single run.
 
 The standard scenarios carry all six faces.  These layouts give tiles whose mask holds
 one face of an axis, and boxes that end at a wall with no slab, which the
@@ -47,17 +49,18 @@ def _present(problem):
     return [f["face"] for f in problem.header["cpml"]]
 
 
+@pytest.mark.parametrize("cells", [1, 2])
 @pytest.mark.parametrize("boundary", ["reference", "matlab"])
 @pytest.mark.parametrize("fuse", [0, 1, 2, 3, 4, 5])
 @pytest.mark.parametrize("layout", sorted(LAYOUTS))
-def test_uneven_layout_fp64_matches_oracle(get_problem, layouts, layout, fuse, boundary):
+def test_uneven_layout_fp64_matches_oracle(get_problem, layouts, layout, fuse, boundary, cells):
     from fdtdhip.sim import HipSim
     from oracle import OracleSim
     prob = get_problem(layouts[layout], steps=STEPS)
     faces = _present(prob)
     assert 0 < len(faces) < 6
     sim = HipSim(prob, boundary=boundary)
-    sim.set_tuning(0, 0, 0, fuse)
+    sim.set_tuning(0, cells, 0, fuse)
     ref = OracleSim(prob, boundary=boundary)
     v, i = sim.run(0, STEPS)
     rv, ri = ref.run(0, STEPS)
@@ -69,6 +72,30 @@ def test_uneven_layout_fp64_matches_oracle(get_problem, layouts, layout, fuse, b
             assert np.array_equal(sim.psi(face, slot), ref.psi(face, slot)), f"Psi {face}/{slot}"
     assert np.abs(sim.field("Ez")).max() > 0
     sim.close()
+
+
+@pytest.mark.parametrize("fuse", [0, 1, 2, 3, 4, 5])
+@pytest.mark.parametrize("cells", [2, 4])
+@pytest.mark.parametrize("layout", sorted(LAYOUTS))
+def test_uneven_layout_fp32_shapes_agree(get_problem, layouts, layout, cells, fuse):
+    """FP32 in every launch shape and placement gives the bits of one-cell vectors with
+    the separate pass: the arithmetic and its order per element are the same."""
+    from fdtdhip.sim import HipSim
+    prob = get_problem(layouts[layout], steps=STEPS, fp32=True)
+    faces = _present(prob)
+    runs = []
+    for tuning in ((1, 1, 0, 0), (3, cells, 0, fuse)):
+        sim = HipSim(prob, precision="fp32")
+        sim.set_tuning(*tuning)
+        v, i = sim.run(0, STEPS)
+        runs.append((v, i, [sim.field(c) for c in ("Hx", "Hy", "Hz", "Ex", "Ey", "Ez")],
+                     [sim.psi(f, s) for f in faces for s in range(4)]))
+        sim.close()
+    (v0, i0, f0, p0), (v1, i1, f1, p1) = runs
+    assert np.array_equal(v0, v1) and np.array_equal(i0, i1)
+    assert all(np.array_equal(a, b) for a, b in zip(f0, f1))
+    assert all(np.array_equal(a, b) for a, b in zip(p0, p1))
+    assert np.abs(f0[5]).max() > 0
 
 
 def test_uneven_layouts_batched_match_single(get_problem, layouts):
