@@ -218,7 +218,7 @@ def ifa_study(workdir, scenario="ifa-config1-7000"):
 
 
 def sweep_study(workdir, count=64, batch=64, seed=4):
-    """Config 4: `count` seeded IFA variants (uneven grids), FP32, `batch` of them stepped
+    """The sweep: `count` seeded IFA variants (uneven grids), FP32, `batch` of them stepped
     together by one launch per phase (tuned by the batch autotuner), 64-step chunks
     replayed from a captured graph."""
     sys.path.insert(0, os.path.join(REPO, "special-topics_amd", "scenarios"))
@@ -264,7 +264,7 @@ EARLYSTOP_WEIGHTS = os.path.join(REPO, "special-topics_amd", "fdtdhip", "earlyst
 
 
 def early_stop_study(count=4, batch_count=64, first=4600):
-    """Config 5 time-to-solution: the full-length FP64 run to S11 against the FP32 run
+    """Early-stop time-to-solution: the full-length FP64 run to S11 against the FP32 run
     stopped at step 1500 + the network (HIP kernels) + S11, with the FP64 fallback when
     the network's S11-error estimate exceeds its calibrated threshold; once with
     `batch_count` variants in one batched launch on each side (fallbacks re-run as their

@@ -1,13 +1,13 @@
-"""BASELINE configurations 2-4 at their own sizes on the GPU.
+"""The benchmark workloads at their own sizes on the GPU.
 
-Config 2: the 39x136x136 IFA (ifa-config1-7000) in FP32 against its FP64 run over all
+The 39x136x136 IFA (ifa-config1-7000) in FP32 against its FP64 run over all
 7000 steps (stable matlab boundary): S11 in dB, the resonance, and the directivity
 cuts; and the 256^3 stencil-roofline case (roofline-256): the FP64 HIP path against the
 threaded CPU oracle bit for bit over its first steps, FP32 against FP64 over 300 steps.
-Config 3: the 512x512x384 patch (patch-512), NTFF at 16 frequencies: the matrix-core
+The 512x512x384 patch (patch-512), NTFF at 16 frequencies: the matrix-core
 blocked NTFF against the direct running DFT, and FP32 against FP64 when the box has
 room for the FP64 problem file.
-Config 4: eight seeded IFA variants from the sweep's grid ranges
+Eight seeded IFA variants from the sweep's grid ranges
 (32-48 x 110-160 x 110-160): each variant's FP32 S11 from the batched launch against
 its own FP64 run, and the batch against single FP32 runs bit for bit.
 The bounds below are what these runs must stay within; the measured values are
@@ -68,7 +68,7 @@ def ifa_outputs(problem, precision):
 def test_config2_ifa_fp32_tracks_fp64_over_7000_steps(get_problem):
     p64 = get_problem(os.path.join(SCEN, "ifa-config1-7000.json"))
     p32 = get_problem(os.path.join(SCEN, "ifa-config1-7000.json"), fp32=True)
-    print("[config 2] ifa-config1-7000 FP64 and FP32, 7000 steps, matlab boundary", flush=True)
+    print("[ifa] ifa-config1-7000 FP64 and FP32, 7000 steps, matlab boundary", flush=True)
     f, S64, d64 = ifa_outputs(p64, "fp64")
     _, S32, d32 = ifa_outputs(p32, "fp32")
     ds11 = float(np.max(np.abs(db(S32) - db(S64))))
@@ -79,7 +79,7 @@ def test_config2_ifa_fp32_tracks_fp64_over_7000_steps(get_problem):
             for q in range(t64.shape[0]):
                 m = t64[q] >= 1e-3 * t64[q].max()
                 ddir = max(ddir, float(np.max(np.abs(10 * np.log10(t32[q][m] / t64[q][m])))))
-    print(f"[config 2] IFA FP32 vs FP64: S11 {ds11:.2e} dB, resonance shift {shift / 1e6:.0f} MHz, "
+    print(f"[ifa] FP32 vs FP64: S11 {ds11:.2e} dB, resonance shift {shift / 1e6:.0f} MHz, "
           f"directivity within 30 dB of peak {ddir:.2e} dB", flush=True)
     assert ds11 <= 1e-3 and shift == 0.0 and ddir <= 1e-3
 
@@ -88,7 +88,7 @@ def test_config2_roofline_256_fp64_matches_oracle(get_problem):
     from fdtdhip.sim import HipSim
     steps = 3
     p = get_problem(os.path.join(SCEN, "roofline-256.json"), steps=8)
-    print("[config 2] roofline-256 FP64: HIP against the threaded oracle, 3 steps", flush=True)
+    print("[roofline-256] FP64: HIP against the threaded oracle, 3 steps", flush=True)
     sim = HipSim(p, precision="fp64")
     try:
         v, i = sim.run(0, steps)
@@ -114,7 +114,7 @@ def test_config2_roofline_256_fp32_tracks_fp64(get_problem):
         finally:
             sim.close()
     err = max(np.max(np.abs(a - b)) / np.max(np.abs(b)) for a, b in zip(out["fp32"], out["fp64"]))
-    print(f"[config 2] roofline-256 FP32 vs FP64 traces over {steps} steps: {err:.2e} of peak", flush=True)
+    print(f"[roofline-256] FP32 vs FP64 traces over {steps} steps: {err:.2e} of peak", flush=True)
     assert err <= 1e-4
 
 
@@ -124,7 +124,7 @@ def test_config3_patch512_blocked_ntff_matches_direct(get_problem):
     p = get_problem(os.path.join(SCEN, "patch-512.json"), steps=steps, fp32=True)
     acc = {}
     for mode in ("direct", "blocked"):
-        print(f"[config 3] patch-512 FP32, {steps} steps, NTFF {mode}", flush=True)
+        print(f"[patch-512] FP32, {steps} steps, NTFF {mode}", flush=True)
         sim = HipSim(p, precision="fp32")
         sim.set_ntff_mode(mode)
         try:
@@ -136,7 +136,7 @@ def test_config3_patch512_blocked_ntff_matches_direct(get_problem):
     for (dr, di), (br, bi) in zip(acc["direct"], acc["blocked"]):
         scale = max(np.max(np.abs(dr)), np.max(np.abs(di)), 1e-300)
         worst = max(worst, np.max(np.abs(br - dr)) / scale, np.max(np.abs(bi - di)) / scale)
-    print(f"[config 3] blocked vs direct NTFF: {worst:.2e} of each accumulator's peak", flush=True)
+    print(f"[patch-512] blocked vs direct NTFF: {worst:.2e} of each accumulator's peak", flush=True)
     assert worst <= 1e-12
 
 
@@ -150,7 +150,7 @@ def test_config3_patch512_fp32_tracks_fp64(get_problem, problem_dir):
     steps = 200
     out = {}
     for prec in ("fp64", "fp32"):
-        print(f"[config 3] patch-512 {prec}, {steps} steps", flush=True)
+        print(f"[patch-512] {prec}, {steps} steps", flush=True)
         p = get_problem(os.path.join(SCEN, "patch-512.json"), steps=steps, fp32=prec == "fp32")
         sim = HipSim(p, precision=prec)
         try:
@@ -158,7 +158,7 @@ def test_config3_patch512_fp32_tracks_fp64(get_problem, problem_dir):
         finally:
             sim.close()
     err = max(np.max(np.abs(a - b)) / np.max(np.abs(b)) for a, b in zip(out["fp32"], out["fp64"]))
-    print(f"[config 3] patch-512 FP32 vs FP64 traces over {steps} steps: {err:.2e} of peak", flush=True)
+    print(f"[patch-512] FP32 vs FP64 traces over {steps} steps: {err:.2e} of peak", flush=True)
     assert err <= 1e-4
 
 
@@ -184,7 +184,7 @@ def test_config4_batched_fp32_variants(problem_dir):
     assert dims[:, 0].min() >= 32 and dims[:, 0].max() <= 48
     assert dims[:, 1:].min() >= 110 and dims[:, 1:].max() <= 160
     f = freq_grid(p32[0])
-    print(f"[config 4] 8 variants, grids {dims.tolist()}, FP32 batched", flush=True)
+    print(f"[ifa-variants] 8 variants, grids {dims.tolist()}, FP32 batched", flush=True)
     sims = [HipSim(p, precision="fp32", boundary="matlab", port_dft_freqs=f, farfield=False) for p in p32]
     multi = HipMulti(sims, cells_per_thread=4)
     multi.run_async(0, p32[0].steps)
@@ -202,7 +202,7 @@ def test_config4_batched_fp32_variants(problem_dir):
             worst = max(worst, float(np.max(np.abs(db(S32[k]) - db(s11(sim, p))))))
         finally:
             sim.close()
-    print(f"[config 4] FP32 batched vs FP64 single S11: max {worst:.2e} dB", flush=True)
+    print(f"[ifa-variants] FP32 batched vs FP64 single S11: max {worst:.2e} dB", flush=True)
     assert worst <= 1e-3
     for k in range(2):                      # the batch equals single FP32 runs
         sim = HipSim(p32[k], precision="fp32", boundary="matlab", port_dft_freqs=f, farfield=False)
@@ -223,7 +223,7 @@ def test_config5_early_stop_pipeline():
     threshold; the printed errors are what the pipeline delivers."""
     from fdtdhip import earlystop
     weights = os.path.join(REPO, "special-topics_amd", "fdtdhip", "earlystop_weights.npz")
-    print("[config 5] early stop on variants 4600..4607, batched", flush=True)
+    print("[early-stop] early stop on variants 4600..4607, batched", flush=True)
     r = earlystop.time_to_solution_batched(weights, 4600, 8)
     pv = r["per_variant"]
     net = earlystop.Predictor(weights)
@@ -234,5 +234,5 @@ def test_config5_early_stop_pipeline():
             assert err == 0.0 and est > th - 1e-4
         else:
             assert est <= th + 1e-4
-    print(f"[config 5] fallbacks {r['fallbacks']}/8, errors {[round(e, 3) for e in pv['max_err_db']]} dB, "
+    print(f"[early-stop] fallbacks {r['fallbacks']}/8, errors {[round(e, 3) for e in pv['max_err_db']]} dB, "
           f"speed-up {r['speedup']}", flush=True)
