@@ -119,6 +119,38 @@ constexpr bool early_tables() {
   return CM == COEF_U8 && (FC == CPML_SEPARATE || FC == CPML_FUSE_XY);
 }
 
+// Cross-lane neighbours by a DPP wavefront shift (an operand modifier of a VALU move, no
+// memory access): with DPP_WAVE_SHL1 each lane receives x from the next lane, with
+// DPP_WAVE_SHR1 from the previous one; the lane at the wavefront's end (63 / 0), whose
+// neighbour is in another wavefront, keeps `edge`.
+constexpr int DPP_WAVE_SHL1 = 0x130, DPP_WAVE_SHR1 = 0x138;
+
+template <int CTRL>
+__device__ __forceinline__ float dpp_shift(float x, float edge) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(edge), __float_as_int(x), CTRL, 0xF, 0xF, false));
+}
+
+template <int CTRL>
+__device__ __forceinline__ double dpp_shift(double x, double edge) {
+  const long long xi = __double_as_longlong(x), ei = __double_as_longlong(edge);
+  const int lo = __builtin_amdgcn_update_dpp((int)ei, (int)xi, CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp((int)(ei >> 32), (int)(xi >> 32), CTRL, 0xF, 0xF, false);
+  return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+
+// Whether a stencil whose lanes hold consecutive W-cell vectors of a row (ROW_LANES: the
+// march, not the z-slab march of CPML_SPLIT_Z, whose lanes pair the ends of two rows)
+// takes the k neighbour just past (H) or before (E) its vector from the adjacent lane
+// instead of loading it.  A scalar load of it spans as many cache lines per
+// wavefront as a vector load (the lanes are W elements apart) for a quarter of the bytes,
+// and the stencils are bound by their memory instructions.  The adjacent lane covers that
+// element whenever it is used: for H the element lies in the same row as a cell with
+// k < nz, whose next lane is then active; for E the previous lane is active whenever this
+// one is (threads leave the march from the end of the column).  The placements that hand
+// slab cells to other workgroups let lanes skip planes, so they keep the loads.
+template <int FC>
+constexpr bool k_neighbour_by_shift() { return FC != CPML_SPLIT_Z && FC != CPML_SPLIT_ALL; }
+
 // W consecutive cells of one row, loaded and stored as one W*sizeof(T)-byte access.
 template <typename T, int W>
 struct alignas(sizeof(T) * W) Vec {
@@ -557,11 +589,12 @@ struct Fields {
 // gives the same result as the separate k_cpml pass.  Every array has a zero guard
 // plane before plane 0 and after plane nx, so all neighbour loads are unconditional;
 // elements outside a component's Yee shape are written back unchanged.
+// ROW_LANES: the lanes hold consecutive W-cell vectors of a row (k_neighbour_by_shift).
 // EARLY (early_tables): the kernel has issued its table loads into *tp and the LDS table
 // is filled here, by table_commit, once the first plane's loads are out.  Every thread of
 // the workgroup calls in, those past the plane's last row (active false) with j = k0 = 0
 // so that their loads stay in bounds; they leave after the barrier.
-template <typename T, int CM, int W, int FC, bool NT = true, bool EARLY = false>
+template <typename T, int CM, int W, int FC, bool NT = true, bool EARLY = false, bool ROW_LANES = true>
 __device__ __forceinline__ void update_h_cells(
     T *__restrict__ Hx, T *__restrict__ Hy, T *__restrict__ Hz,
     const T *__restrict__ Ex, const T *__restrict__ Ey, const T *__restrict__ Ez,
@@ -592,7 +625,14 @@ __device__ __forceinline__ void update_h_cells(
     }
     const V ex_i = ldv<T, W>(Ex + c);
     const V ez_j = ldv<T, W>(Ez + c + SJ), ex_j = ldv<T, W>(Ex + c + SJ);
-    const T ey_w = Ey[c + W], ex_w = Ex[c + W];           // first cell past the vector
+    // First cell past the vector: from the next lane (lane 63 loads it).
+    constexpr bool SHIFT = ROW_LANES && k_neighbour_by_shift<FC>();
+    const bool edge = !SHIFT || (threadIdx.x & 63) == 63;
+    T ey_w = T(0), ex_w = T(0);
+    if (edge) {
+      ey_w = Ey[c + W];
+      ex_w = Ex[c + W];
+    }
     V hx = ldv_stream<T, W, NT>(Hx + c), hy = ldv_stream<T, W, NT>(Hy + c), hz = ldv_stream<T, W, NT>(Hz + c);
     V c0, c1, c2, c3, c4, c5, c6, c7, c8;
     if (EARLY) {
@@ -609,6 +649,10 @@ __device__ __forceinline__ void update_h_cells(
       coeffs<T, CM, W, NT>(g.ct, tc, 0, 0, c, g.C[CHXH], g.C[CHXEY], g.C[CHXEZ], c0, c1, c2);
       coeffs<T, CM, W, NT>(g.ct, tc, 0, 1, c, g.C[CHYH], g.C[CHYEZ], g.C[CHYEX], c3, c4, c5);
       coeffs<T, CM, W, NT>(g.ct, tc, 0, 2, c, g.C[CHZH], g.C[CHZEX], g.C[CHZEY], c6, c7, c8);
+    }
+    if constexpr (SHIFT) {
+      ey_w = dpp_shift<DPP_WAVE_SHL1>(ey_i[0], ey_w);
+      ex_w = dpp_shift<DPP_WAVE_SHL1>(ex_i[0], ex_w);
     }
     // The six E differences of the curls; the CPML terms use the same ones (E does not
     // change during the H phase).
@@ -760,7 +804,7 @@ __device__ __forceinline__ void e_wall_terms(const Grid<T> &g, const FaceSet<T> 
 // every face fused (FC == CPML_FUSE_ALL) the same thread then applies the wall's CPML
 // terms, which read only H.  Elements the bulk does not update keep their value.
 // EARLY, tp and active: as for update_h_cells.
-template <typename T, int CM, int W, int FC, bool NT = true, bool EARLY = false>
+template <typename T, int CM, int W, int FC, bool NT = true, bool EARLY = false, bool ROW_LANES = true>
 __device__ __forceinline__ void update_e_cells(
     T *__restrict__ Ex, T *__restrict__ Ey, T *__restrict__ Ez,
     const T *__restrict__ Hx, const T *__restrict__ Hy, const T *__restrict__ Hz,
@@ -794,7 +838,14 @@ __device__ __forceinline__ void update_e_cells(
       continue;
     }
     const V hz_j = ldv<T, W>(Hz + c - SJ), hx_j = ldv<T, W>(Hx + c - SJ);
-    const T hy_w = Hy[c - 1], hx_w = Hx[c - 1];            // last cell before the vector
+    // Last cell before the vector: from the previous lane (lane 0 loads it).
+    constexpr bool SHIFT = ROW_LANES && k_neighbour_by_shift<FC>();
+    const bool edge = !SHIFT || (threadIdx.x & 63) == 0;
+    T hy_w = T(0), hx_w = T(0);
+    if (edge) {
+      hy_w = Hy[c - 1];
+      hx_w = Hx[c - 1];
+    }
     V ex = ldv_stream<T, W, NT>(Ex + c), ey = ldv_stream<T, W, NT>(Ey + c), ez = ldv_stream<T, W, NT>(Ez + c);
     V c0, c1, c2, c3, c4, c5, c6, c7, c8;
     if (EARLY) {
@@ -811,6 +862,10 @@ __device__ __forceinline__ void update_e_cells(
       coeffs<T, CM, W, NT>(g.ct, tc, 3, 0, c, g.C[CEXE], g.C[CEXHZ], g.C[CEXHY], c0, c1, c2);
       coeffs<T, CM, W, NT>(g.ct, tc, 3, 1, c, g.C[CEYE], g.C[CEYHX], g.C[CEYHZ], c3, c4, c5);
       coeffs<T, CM, W, NT>(g.ct, tc, 3, 2, c, g.C[CEZE], g.C[CEZHY], g.C[CEZHX], c6, c7, c8);
+    }
+    if constexpr (SHIFT) {
+      hy_w = dpp_shift<DPP_WAVE_SHR1>(hy_i[W - 1], hy_w);
+      hx_w = dpp_shift<DPP_WAVE_SHR1>(hx_i[W - 1], hx_w);
     }
     // The six H differences of the curls, shared with the CPML terms (H does not change
     // during the E phase).
@@ -949,7 +1004,7 @@ __global__ void __launch_bounds__(256) k_update_h_split(Grid<T> g, MarchGeom m, 
     load_tables<T, CM>(g, 0, tc);
     int j, k;
     if (!zslab_lane(z, col * 256 + (int)threadIdx.x, j, k)) return;
-    update_h_cells<T, CM, 1, CPML_FUSE_Z>(g.Hx, g.Hy, g.Hz, g.Ex, g.Ey, g.Ez, g, tc, j, k, ch * z.chunk, z.chunk, fs);
+    update_h_cells<T, CM, 1, CPML_FUSE_Z, true, false, false>(g.Hx, g.Hy, g.Hz, g.Ex, g.Ey, g.Ez, g, tc, j, k, ch * z.chunk, z.chunk, fs);
   } else {
     if (!march_tile_at(m, L - z.ntiles, col, ch)) return;
     load_tables<T, CM>(g, 0, tc);
@@ -967,7 +1022,7 @@ __global__ void __launch_bounds__(256) k_update_e_split(Grid<T> g, MarchGeom m, 
     load_tables<T, CM>(g, 3, tc);
     int j, k;
     if (!zslab_lane(z, col * 256 + (int)threadIdx.x, j, k)) return;
-    update_e_cells<T, CM, 1, CPML_FUSE_Z>(g.Ex, g.Ey, g.Ez, g.Hx, g.Hy, g.Hz, g, tc, j, k, ch * z.chunk, z.chunk,
+    update_e_cells<T, CM, 1, CPML_FUSE_Z, true, false, false>(g.Ex, g.Ey, g.Ez, g.Hx, g.Hy, g.Hz, g, tc, j, k, ch * z.chunk, z.chunk,
                                           ghost, fs);
   } else {
     if (!march_tile_at(m, L - z.ntiles, col, ch)) return;
