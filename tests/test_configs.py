@@ -102,6 +102,59 @@ def test_config2_roofline_256_fp64_matches_oracle(get_problem):
         assert np.array_equal(a, ref.field(c)), c
 
 
+def _random_state(problem, seed):
+    """Seeded uniform fields (Hx..Ez) and Psi arrays of every CPML face slot."""
+    rng = np.random.default_rng(seed)
+    fields = {c: rng.uniform(-1.0, 1.0, problem.component_len(c)) for c in ("Hx", "Hy", "Hz", "Ex", "Ey", "Ez")}
+    slot_comp = {"x": ["Hy", "Hz", "Ey", "Ez"], "y": ["Hx", "Hz", "Ez", "Ex"], "z": ["Hx", "Hy", "Ex", "Ey"]}
+    psi = {}
+    for f in problem.header["cpml"]:
+        face = f["face"]
+        for slot in range(4):
+            d = list(problem.component_dims(slot_comp[face[0]][slot]))
+            d["xyz".index(face[0])] = f["nc"]
+            psi[(face, slot)] = rng.uniform(-1.0, 1.0, d[0] * d[1] * d[2])
+    return fields, psi
+
+
+def test_config2_roofline_256_random_state_fp64_matches_oracle(get_problem):
+    """Seeded random fields and Psi over the whole 256^3 grid, then two steps: every cell,
+    every CPML slot and every tile and column boundary of the full-size launches, in the
+    separate pass and in the fused placements, bit for bit against the threaded oracle.
+    (From rest, the first steps leave the CPML slabs untouched.)"""
+    from fdtdhip.sim import HipSim
+    steps = 2
+    p = get_problem(os.path.join(SCEN, "roofline-256.json"), steps=8)
+    fields, psi = _random_state(p, 11)
+    print("[roofline-256] FP64 random state: threaded oracle, 2 steps", flush=True)
+    ref = OracleSim(p, threads=min(16, os.cpu_count() or 1))
+    for c, a in fields.items():
+        ref.field(c)[:] = a
+    for (face, slot), a in psi.items():
+        ref.psi(face, slot)[:] = a
+    rv, ri = ref.run(0, steps)
+    want_f = {c: ref.field(c).copy() for c in fields}
+    want_p = {k: ref.psi(*k).copy() for k in psi}
+    del ref
+    for tuning in ((1, 2, 0, 0), (2, 1, 0, 5), (4, 2, 0, 1), (2, 2, 1, 2)):
+        print(f"[roofline-256] FP64 random state: HIP {tuning}", flush=True)
+        sim = HipSim(p, precision="fp64")
+        try:
+            sim.set_tuning(*tuning)
+            for c, a in fields.items():
+                sim.set_field(c, a)
+            for (face, slot), a in psi.items():
+                sim.set_psi(face, slot, a)
+            v, i = sim.run(0, steps)
+            assert np.array_equal(v, rv.T) and np.array_equal(i, ri.T), tuning
+            for c, a in want_f.items():
+                assert np.array_equal(sim.field(c), a), (tuning, c)
+            for (face, slot), a in want_p.items():
+                assert np.array_equal(sim.psi(face, slot), a), (tuning, face, slot)
+        finally:
+            sim.close()
+
+
 def test_config2_roofline_256_fp32_tracks_fp64(get_problem):
     from fdtdhip.sim import HipSim
     steps = 300
