@@ -215,6 +215,50 @@ def test_config3_patch512_fp32_tracks_fp64(get_problem, problem_dir):
     assert err <= 1e-4
 
 
+@pytest.mark.timeout(900)
+def test_config3_patch512_random_state_fp64_matches_oracle(get_problem, problem_dir):
+    """Seeded random fields and Psi over the whole 512x512x384 patch, then one step, bit for
+    bit against the threaded oracle: traces, fields, every Psi and the 24 NTFF accumulators
+    (direct running DFT at 16 frequencies), in two launch shapes.  Uses the FP64 problem
+    file of the test above (same disk and host-RAM needs; the oracle holds about 20 GB)."""
+    from fdtdhip.sim import HipSim
+    if shutil.disk_usage(problem_dir).free < 40e9:
+        pytest.skip("fewer than 40 GB free for the 512x512x384 FP64 problem file")
+    p = get_problem(os.path.join(SCEN, "patch-512.json"), steps=200)
+    fields, psi = _random_state(p, 13)
+    print("[patch-512] FP64 random state: threaded oracle, 1 step", flush=True)
+    ref = OracleSim(p, threads=min(16, os.cpu_count() or 1))
+    for c, a in fields.items():
+        ref.field(c)[:] = a
+    for (face, slot), a in psi.items():
+        ref.psi(face, slot)[:] = a
+    rv, ri = ref.run(0, 1)
+    want_f = {c: ref.field(c).copy() for c in fields}
+    want_p = {k: ref.psi(*k).copy() for k in psi}
+    want_ff = [(ref.ff(q, False).copy(), ref.ff(q, True).copy()) for q in range(24)]
+    del ref
+    for tuning in ((1, 2, 0, 0), (2, 1, 0, 5)):
+        print(f"[patch-512] FP64 random state: HIP {tuning}", flush=True)
+        sim = HipSim(p, precision="fp64")
+        try:
+            sim.set_tuning(*tuning)
+            for c, a in fields.items():
+                sim.set_field(c, a)
+            for (face, slot), a in psi.items():
+                sim.set_psi(face, slot, a)
+            v, i = sim.run(0, 1)
+            assert np.array_equal(v, rv.T) and np.array_equal(i, ri.T), tuning
+            for c, a in want_f.items():
+                assert np.array_equal(sim.field(c), a), (tuning, c)
+            for (face, slot), a in want_p.items():
+                assert np.array_equal(sim.psi(face, slot), a), (tuning, face, slot)
+            for q, (re, im) in enumerate(want_ff):
+                gr, gi = sim.farfield(q)
+                assert np.array_equal(gr, re) and np.array_equal(gi, im), (tuning, "ntff", q)
+        finally:
+            sim.close()
+
+
 def test_config4_batched_fp32_variants(problem_dir):
     from fdtdhip.sim import HipMulti, HipSim
     from fdtdhip.sweep import build_all
