@@ -39,3 +39,29 @@ def test_directivity_at_equals_its_row_of_every_frequency(get_problem, precision
                 sim.directivity_at(f, th, ph)
     finally:
         sim.close()
+
+
+def test_directivity_bits_do_not_depend_on_the_angle_count(get_problem):
+    """Each angle's sums run over the surface cells in one order whatever the other angles of
+    the call are (its block's group of four included), so a prefix of the angles, down to a
+    partial group, gives the bits of the full call, and one frequency those of its row."""
+    from fdtdhip.sim import HipSim
+    prob = get_problem("golden-small")
+    sim = HipSim(prob, precision="fp64")
+    try:
+        sim.run(0, prob.steps)
+        rng = np.random.default_rng(11)
+        th = rng.uniform(0.0, np.pi, 4100)
+        ph = rng.uniform(0.0, 2 * np.pi, 4100)
+        prad = sim.radiated_power()
+        assert len(prad) == 3
+        want_t, want_p = sim.directivity(th, ph, prad)
+        for n in (1000, 601, 5):
+            print(f"directivity: golden-small fp64, {n} of 4100 angles", flush=True)
+            got_t, got_p = sim.directivity(th[:n], ph[:n], prad)
+            assert np.array_equal(got_t, want_t[:, :n]) and np.array_equal(got_p, want_p[:, :n]), n
+        for f in range(3):
+            got_t, got_p = sim.directivity_at(f, th[:720], ph[:720], prad[f])
+            assert np.array_equal(got_t, want_t[f, :720]) and np.array_equal(got_p, want_p[f, :720]), f
+    finally:
+        sim.close()
