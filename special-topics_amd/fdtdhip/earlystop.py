@@ -104,7 +104,7 @@ def s11_from(V, I, z=50.0):
 
 # ------------------------------------------------------------------------ data --
 
-def _scenarios(first, count, seed, workdir, steps=STEPS):
+def variant_scenarios(first, count, seed, workdir, steps=STEPS):
     here = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
     sys.path.insert(0, os.path.join(here, "scenarios"))
     import gen_ifa_variants
@@ -124,7 +124,7 @@ def gen_train(first, count, out, seed=4, workdir="/tmp/es-data", batch=64, jobs=
     """FP32 full runs (matlab boundary) of variants [first, first+count), `batch` at a
     time in one batched launch: network input and targets, plus what the S11-error
     target needs -- the full-rate prefix traces and the full run's S11."""
-    paths, params = _scenarios(first, count, seed, workdir)
+    paths, params = variant_scenarios(first, count, seed, workdir)
     builds = LazyBuilds(paths, workdir, fp32=True, jobs=jobs, ahead=2 * batch)
     res, secs, cells = Sweep("fp32", "matlab", keep_traces=True, farfield=False, s11=True,
                              progress_every=1, batch=batch).run(builds)
@@ -146,7 +146,7 @@ def gen_train(first, count, out, seed=4, workdir="/tmp/es-data", batch=64, jobs=
 def gen_heldout(first, count, out, seed=4, workdir="/tmp/es-heldout", batch=64, jobs=16):
     """Per held-out variant: the full-length FP64 S11 (the reference result) and the FP32 prefix
     traces at full rate (what an early-stopped run sees)."""
-    paths, params = _scenarios(first, count, seed, workdir)
+    paths, params = variant_scenarios(first, count, seed, workdir)
     b64 = LazyBuilds(paths, os.path.join(workdir, "f64"), fp32=False, jobs=jobs, ahead=2 * batch)
     res64, secs64, cells64 = Sweep("fp64", "matlab", keep_traces=False, farfield=False,
                                    progress_every=1, batch=batch).run(b64)
@@ -263,7 +263,7 @@ def time_to_solution(weights_path, first, count, seed=4, workdir="/tmp/es-tts"):
     from .problem import read_problem
     from .sim import HipSim
     from .sweep import build_one, s11
-    paths, _ = _scenarios(first, count, seed, workdir)
+    paths, _ = variant_scenarios(first, count, seed, workdir)
     freqs = frequency_grid({"maxFreq": 6e9, "step": 6e6})
     net = Predictor(weights_path)
     rows = []
@@ -336,7 +336,7 @@ def time_to_solution_batched(weights_path, first, count, seed=4, workdir="/tmp/e
     variant."""
     from .problem import read_problem
     from .sim import HipMulti, HipSim
-    paths, _ = _scenarios(first, count, seed, workdir)
+    paths, _ = variant_scenarios(first, count, seed, workdir)
     freqs = frequency_grid({"maxFreq": 6e9, "step": 6e6})
     p64 = [read_problem(f) for f in build_all(paths, os.path.join(workdir, "f64"), fp32=False, jobs=jobs)]
     p32 = [read_problem(f) for f in build_all(paths, os.path.join(workdir, "f32"), fp32=True, jobs=jobs)]

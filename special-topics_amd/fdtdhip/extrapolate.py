@@ -81,7 +81,7 @@ def study(first, count, prefixes=(1500, 2000, 2500, 3000, 3500, 4000), seed=4, b
           workdir="/tmp/es-extrapolate", start=1100, dec=8, tol=1e-6):
     from . import earlystop as es
     from .sweep import LazyBuilds, Sweep, frequency_grid
-    paths, _ = es._scenarios(first, count, seed, workdir)
+    paths, _ = es.variant_scenarios(first, count, seed, workdir)
     runs, secs = {}, {}
     for prec in ("fp64", "fp32"):
         builds = LazyBuilds(paths, f"{workdir}/{prec}", fp32=prec == "fp32", jobs=16, ahead=2 * batch)
